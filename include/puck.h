@@ -1,0 +1,137 @@
+/*
+ * puck-mi355x C ABI: the drop-in boundary of the CKY parse path.
+ *
+ * Every entry point returns 0 on success and a negative status on failure; the message of the
+ * last failure on the calling thread is returned by puck_last_error(). Host buffers belong to the
+ * caller, device memory belongs to the library. A parser handle is internally serialised (the
+ * reference's `synchronized` methods); distinct handles are independent and own one HIP stream
+ * each.
+ *
+ * Reference interfaces replaced (Scala/JavaCL, citations relative to the reference checkout):
+ *   CLParserData.make / readSequence      src/main/scala/puck/parser/CLParser.scala:1264-1305
+ *   new CLParser(data, maxAllocSize)      src/main/scala/puck/parser/CLParser.scala:46-48
+ *   CLParser.parse                        src/main/scala/puck/parser/CLParser.scala:50-53
+ *   CLParser.partitions                   src/main/scala/puck/parser/CLParser.scala:55-58
+ *   CLParser.insideOutside                src/main/scala/puck/parser/CLParser.scala:60-63
+ *   ActualParser.updateMasks (coarse)     src/main/scala/puck/parser/CLParser.scala:208-229
+ */
+#ifndef PUCK_H_
+#define PUCK_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct puck_grammar puck_grammar_t;
+typedef struct puck_parser puck_parser_t;
+
+enum puck_semiring { PUCK_VITERBI = 0, PUCK_REAL = 1 };
+
+/* Status codes. */
+enum puck_status {
+  PUCK_OK = 0,
+  PUCK_ERR_ARG = -1,      /* bad argument */
+  PUCK_ERR_IO = -2,       /* grammar file missing or malformed */
+  PUCK_ERR_DEVICE = -3,   /* HIP runtime error */
+  PUCK_ERR_NOMEM = -4,    /* allocation failure (host or device) */
+  PUCK_ERR_INTERNAL = -5
+};
+
+/* Pre-order tree node of a Viterbi parse, 2n-1 per sentence (CLViterbi.scala:378 tree_t). */
+typedef struct {
+  int32_t top_sym; /* nonterminal id of the unary top, -1 when the node skips the unary layer */
+  int32_t bot_sym; /* nonterminal id (width > 1) or terminal id (width == 1) */
+  int32_t width;
+  float score;
+} puck_tree_node_t;
+
+/* Pre-order node of an MBR (max-rule) parse with coarse labels (CLParser.scala:660-683). */
+typedef struct {
+  int32_t top_coarse; /* -1 when the unary layer is not used */
+  int32_t bot_coarse;
+  int32_t begin, end;
+  float score;
+} puck_mbr_node_t;
+
+typedef struct {
+  int32_t num_nonterms, num_terms, num_coarse, cell_size, mask_words, root;
+  int32_t num_binary[4]; /* NN, NT, TN, TT */
+  int32_t num_unary_n, num_unary_t, num_words, semiring;
+  int32_t num_blocks[4]; /* CoarseParent blocks of the inside families */
+  int32_t rules_dropped;
+} puck_grammar_info_t;
+
+typedef struct {
+  double rule_kernel_ms;  /* summed device time of the rule-family kernels */
+  double reduce_ms, other_ms, total_ms;
+  double rule_apps;       /* rule applications executed by the rule kernels */
+  int64_t rule_launches;
+} puck_stats_t;
+
+/* Thread-local message of the last failure ("" if none). */
+const char* puck_last_error(void);
+int32_t puck_abi_version(void);
+
+/* ---- grammars (CLParserData.make over SimpleGrammar.parseBerkeleyText(prefix, -12, ...)) ---- */
+int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out);
+void puck_grammar_free(puck_grammar_t* g);
+int32_t puck_grammar_info(const puck_grammar_t* g, puck_grammar_info_t* out);
+/* Word id of `word` in the grammar lexicon, -1 if unknown. */
+int32_t puck_grammar_word_id(const puck_grammar_t* g, const char* word);
+/* Rule tables for cross-checks. family: 0..3 binary NN/NT/TN/TT (p,l,r), 4 unary N, 5 unary T
+ * (p,c,-). Writes up to `cap` rows of 3 ints plus score w and kernel literal q. Returns the row
+ * count (or a negative status). */
+int32_t puck_grammar_rules(const puck_grammar_t* g, int32_t family, int32_t* idx, float* w, float* q,
+                           int32_t cap);
+/* Symbol table: coarse id and substate of nonterminal (terminal=0) or terminal (terminal=1) ids. */
+int32_t puck_grammar_symbols(const puck_grammar_t* g, int32_t terminal, int32_t* coarse, int32_t* sub,
+                             int32_t cap);
+/* CoarseParent blocks: which = 0 inside, 1 outside-left, 2 outside-right; returns #blocks and
+ * writes block_of_coarse[num_coarse]. */
+int32_t puck_grammar_blocks(const puck_grammar_t* g, int32_t which, int32_t family,
+                            int32_t* block_of_coarse);
+/* Lexicon of one word: writes (terminal id, ln P) pairs, returns count. */
+int32_t puck_grammar_lexicon(const puck_grammar_t* g, int32_t word, int32_t* tags, double* logp,
+                             int32_t cap);
+
+/* ---- seeded inputs ---- */
+/* Synthetic grammar with the wsj_6 shape; writes out_prefix.{numstates,binary,unary,lexicon}. */
+int32_t puck_synth_wsj6(const char* data_dir, const char* out_prefix, uint64_t seed);
+/* PCFG sentence sampling; mode 0 range filter, 1 log-normal lengths, 2 mixed long tail.
+ * words_out needs capacity words_cap; offsets_out needs n+1 entries. */
+int32_t puck_sample_sentences(const puck_grammar_t* g, uint64_t seed, int32_t n, int32_t mode,
+                              int32_t min_len, int32_t max_len, int32_t* words_out, int64_t words_cap,
+                              int32_t* offsets_out);
+
+/* ---- parser (CLParser over coarse-to-fine layers; the last layer decides Viterbi vs MBR) ---- */
+int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device,
+                           int64_t mem_bytes, int32_t flags, puck_parser_t** out);
+void puck_parser_free(puck_parser_t* p);
+/* Sentences are word ids: sentence i is words[offsets[i] .. offsets[i+1]). */
+int32_t puck_parse_viterbi(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
+                           puck_tree_node_t* nodes, int64_t nodes_cap);
+int32_t puck_parse_mbr(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
+                       puck_mbr_node_t* nodes, int64_t nodes_cap);
+int32_t puck_partitions(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
+                        float* out);
+int32_t puck_inside_outside(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n);
+/* Pruning masks of the last coarse layer: mask_words uint32 per chart cell, cells of sentence i at
+ * [sum_{j<i} n_j(n_j+1), ...), bottom half then top half, span-diagonal-major. */
+int32_t puck_masks(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
+                   uint32_t* out, int64_t cap);
+/* Charts of layer `layer` after its passes on one batch (parity tests). Sizes are
+ * cells*cell_size floats for the charts and cells for the scales; masks (cells*mask_words) are the
+ * input masks of that layer when it has any. partition_scales has n entries. */
+int32_t puck_debug_charts(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
+                          int32_t layer, int32_t with_outside, float* inside, float* outside,
+                          float* in_scale, float* out_scale, uint32_t* masks,
+                          double* partition_scales, int64_t chart_cap);
+int32_t puck_parser_stats(const puck_parser_t* p, puck_stats_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PUCK_H_ */

@@ -1,0 +1,66 @@
+"""Builds puck_amd/libpuck.so (gfx950 device code + host engine) with hipcc, in-tree.
+
+Usage: python -m puck_amd.build  (or puck_amd.build.build_library() from Python).
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpuck.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PUCK_OFFLOAD_ARCH", "gfx950")
+
+HOST_SOURCES = ["grammar.cpp", "synth.cpp", "capi.cpp"]
+DEVICE_SOURCES = ["engine.hip"]
+
+COMMON_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]
+# Packed FP32 instructions are kept out of every device build; contraction is off so that rule
+# arithmetic keeps the reference's association and rounding.
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+]
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+    if res.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def build_library(verbose=False):
+    """Compiles every source and links libpuck.so next to this file. Always rebuilds."""
+    tmp = tempfile.mkdtemp(prefix="puck_build_")
+    try:
+        objs = []
+        for src in HOST_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            _run([HIPCC, "-x", "c++", *COMMON_FLAGS, "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                  "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        for src in DEVICE_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            out = _run([HIPCC, *COMMON_FLAGS, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            if verbose and out.strip():
+                print(out)
+            objs.append(obj)
+        tmp_lib = LIB + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp_lib])
+        os.replace(tmp_lib, LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build_library(verbose="-v" in sys.argv))

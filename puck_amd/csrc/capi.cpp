@@ -1,0 +1,325 @@
+// extern "C" boundary over the grammar compiler, input generators and the CKY engine.
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/puck.h"
+#include "engine.h"
+#include "synth.h"
+
+struct puck_grammar {
+  std::shared_ptr<puck::Grammar> g;
+};
+
+struct puck_parser {
+  std::unique_ptr<puck::Parser> p;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t set_error(int32_t code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int32_t guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    return f();
+  } catch (const std::bad_alloc&) {
+    return set_error(PUCK_ERR_NOMEM, "allocation failed");
+  } catch (const std::runtime_error& e) {
+    const std::string m = e.what();
+    if (m.rfind("HIP error", 0) == 0) {
+      if (m.find("out of memory") != std::string::npos) return set_error(PUCK_ERR_NOMEM, m);
+      return set_error(PUCK_ERR_DEVICE, m);
+    }
+    if (m.rfind("cannot open", 0) == 0 || m.rfind("bad ", 0) == 0 || m.rfind("missing", 0) == 0)
+      return set_error(PUCK_ERR_IO, m);
+    return set_error(PUCK_ERR_ARG, m);
+  } catch (const std::exception& e) {
+    return set_error(PUCK_ERR_INTERNAL, e.what());
+  }
+}
+
+int64_t nodes_needed(const int32_t* offsets, int32_t n) {
+  int64_t t = 0;
+  for (int32_t i = 0; i < n; ++i) t += 2 * (int64_t)(offsets[i + 1] - offsets[i]) - 1;
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* puck_last_error(void) { return g_last_error.c_str(); }
+int32_t puck_abi_version(void) { return 1; }
+
+int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out) {
+  return guarded([&]() -> int32_t {
+    if (!prefix || !out) return set_error(PUCK_ERR_ARG, "null argument");
+    if (semiring != PUCK_VITERBI && semiring != PUCK_REAL) return set_error(PUCK_ERR_ARG, "unknown semiring");
+    auto* h = new puck_grammar;
+    try {
+      h->g = std::make_shared<puck::Grammar>(puck::load_text_grammar(prefix, (puck::Semiring)semiring));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return PUCK_OK;
+  });
+}
+
+void puck_grammar_free(puck_grammar_t* g) { delete g; }
+
+int32_t puck_grammar_info(const puck_grammar_t* h, puck_grammar_info_t* out) {
+  return guarded([&]() -> int32_t {
+    if (!h || !out) return set_error(PUCK_ERR_ARG, "null argument");
+    const puck::Grammar& g = *h->g;
+    std::memset(out, 0, sizeof(*out));
+    out->num_nonterms = g.num_nonterms();
+    out->num_terms = g.num_terms();
+    out->num_coarse = (int32_t)g.coarse_names.size();
+    out->cell_size = g.cell_size;
+    out->mask_words = g.mask_words;
+    out->root = g.root;
+    for (int f = 0; f < 4; ++f) {
+      out->num_binary[f] = (int32_t)g.binaries[f].size();
+      out->num_blocks[f] = g.inside_blocks[f].num_blocks;
+    }
+    out->num_unary_n = (int32_t)g.unary_n.size();
+    out->num_unary_t = (int32_t)g.unary_t.size();
+    out->num_words = (int32_t)g.words.size();
+    out->semiring = g.semiring;
+    out->rules_dropped = g.num_rules_dropped;
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_grammar_word_id(const puck_grammar_t* h, const char* word) {
+  if (!h || !word) return -1;
+  auto it = h->g->word_ids.find(word);
+  return it == h->g->word_ids.end() ? -1 : it->second;
+}
+
+int32_t puck_grammar_rules(const puck_grammar_t* h, int32_t family, int32_t* idx, float* w, float* q,
+                           int32_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h) return set_error(PUCK_ERR_ARG, "null grammar");
+    const puck::Grammar& g = *h->g;
+    int32_t n = 0;
+    if (family >= 0 && family < 4) {
+      const auto& v = g.binaries[family];
+      for (const auto& r : v) {
+        if (n < cap) {
+          if (idx) { idx[3 * n] = r.p; idx[3 * n + 1] = r.l; idx[3 * n + 2] = r.r; }
+          if (w) w[n] = r.w;
+          if (q) q[n] = r.q;
+        }
+        ++n;
+      }
+    } else if (family == 4 || family == 5) {
+      const auto& v = family == 4 ? g.unary_n : g.unary_t;
+      for (const auto& r : v) {
+        if (n < cap) {
+          if (idx) { idx[3 * n] = r.p; idx[3 * n + 1] = r.c; idx[3 * n + 2] = -1; }
+          if (w) w[n] = r.w;
+          if (q) q[n] = r.q;
+        }
+        ++n;
+      }
+    } else {
+      return set_error(PUCK_ERR_ARG, "family must be in [0, 5]");
+    }
+    return n;
+  });
+}
+
+int32_t puck_grammar_symbols(const puck_grammar_t* h, int32_t terminal, int32_t* coarse, int32_t* sub,
+                             int32_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h) return set_error(PUCK_ERR_ARG, "null grammar");
+    const auto& v = terminal ? h->g->terms : h->g->nonterms;
+    for (int32_t i = 0; i < (int32_t)v.size() && i < cap; ++i) {
+      if (coarse) coarse[i] = v[i].coarse;
+      if (sub) sub[i] = v[i].sub;
+    }
+    return (int32_t)v.size();
+  });
+}
+
+int32_t puck_grammar_blocks(const puck_grammar_t* h, int32_t which, int32_t family, int32_t* boc) {
+  return guarded([&]() -> int32_t {
+    if (!h || family < 0 || family > 3 || which < 0 || which > 2) return set_error(PUCK_ERR_ARG, "bad argument");
+    const puck::Grammar& g = *h->g;
+    const puck::BlockPartition& bp =
+        which == 0 ? g.inside_blocks[family] : which == 1 ? g.outside_l_blocks[family] : g.outside_r_blocks[family];
+    if (boc) std::memcpy(boc, bp.block_of_coarse.data(), bp.block_of_coarse.size() * 4);
+    return bp.num_blocks;
+  });
+}
+
+int32_t puck_grammar_lexicon(const puck_grammar_t* h, int32_t word, int32_t* tags, double* logp, int32_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h) return set_error(PUCK_ERR_ARG, "null grammar");
+    const puck::Grammar& g = *h->g;
+    if (word < 0 || word + 1 >= (int32_t)g.lex_offsets.size()) return set_error(PUCK_ERR_ARG, "word id out of range");
+    int32_t n = 0;
+    for (int e = g.lex_offsets[word]; e < g.lex_offsets[word + 1]; ++e, ++n)
+      if (n < cap) {
+        if (tags) tags[n] = g.lex_tags[e];
+        if (logp) logp[n] = g.lex_logp[e];
+      }
+    return n;
+  });
+}
+
+int32_t puck_synth_wsj6(const char* data_dir, const char* out_prefix, uint64_t seed) {
+  return guarded([&]() -> int32_t {
+    if (!data_dir || !out_prefix) return set_error(PUCK_ERR_ARG, "null argument");
+    g_last_error = puck::synth_wsj6(data_dir, out_prefix, seed);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_sample_sentences(const puck_grammar_t* h, uint64_t seed, int32_t n, int32_t mode,
+                              int32_t min_len, int32_t max_len, int32_t* words_out, int64_t words_cap,
+                              int32_t* offsets_out) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words_out || !offsets_out || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<int32_t> w, o;
+    if (!puck::sample_sentences(*h->g, seed, n, mode, min_len, max_len, &w, &o))
+      return set_error(PUCK_ERR_INTERNAL, "sentence sampler exhausted its attempt budget");
+    if ((int64_t)w.size() > words_cap) return set_error(PUCK_ERR_ARG, "words_out too small");
+    std::memcpy(words_out, w.data(), w.size() * 4);
+    std::memcpy(offsets_out, o.data(), o.size() * 4);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device,
+                           int64_t mem_bytes, int32_t flags, puck_parser_t** out) {
+  return guarded([&]() -> int32_t {
+    if (!layers || n_layers < 1 || !out) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<std::shared_ptr<puck::Grammar>> gs;
+    for (int32_t i = 0; i < n_layers; ++i) {
+      if (!layers[i]) return set_error(PUCK_ERR_ARG, "null grammar layer");
+      gs.push_back(layers[i]->g);
+    }
+    const int32_t ipl = (flags & 1) ? 1 : 2;
+    auto* h = new puck_parser;
+    try {
+      h->p.reset(new puck::Parser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, ipl));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return PUCK_OK;
+  });
+}
+
+void puck_parser_free(puck_parser_t* p) { delete p; }
+
+int32_t puck_parse_viterbi(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
+                           puck_tree_node_t* nodes, int64_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || !nodes || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    if (nodes_needed(offsets, n) > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
+    std::vector<std::vector<puck::TreeNode>> trees;
+    h->p->parse(words, offsets, n, &trees, nullptr);
+    int64_t o = 0;
+    for (auto& t : trees) {
+      std::memcpy(nodes + o, t.data(), t.size() * sizeof(puck_tree_node_t));
+      o += (int64_t)t.size();
+    }
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_parse_mbr(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
+                       puck_mbr_node_t* nodes, int64_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || !nodes || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    if (nodes_needed(offsets, n) > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
+    std::vector<std::vector<puck::MbrNode>> trees;
+    h->p->parse(words, offsets, n, nullptr, &trees);
+    int64_t o = 0;
+    for (auto& t : trees) {
+      std::memcpy(nodes + o, t.data(), t.size() * sizeof(puck_mbr_node_t));
+      o += (int64_t)t.size();
+    }
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_partitions(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n, float* out) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || !out || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    h->p->partitions(words, offsets, n, out);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_inside_outside(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    h->p->inside_outside(words, offsets, n);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_masks(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n, uint32_t* out,
+                   int64_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || !out || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<uint32_t> m;
+    std::vector<int32_t> co;
+    h->p->masks(words, offsets, n, &m, &co);
+    if ((int64_t)m.size() > cap) return set_error(PUCK_ERR_ARG, "mask buffer too small");
+    std::memcpy(out, m.data(), m.size() * 4);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_debug_charts(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
+                          int32_t layer, int32_t with_outside, float* inside, float* outside,
+                          float* in_scale, float* out_scale, uint32_t* masks, double* partition_scales,
+                          int64_t chart_cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || n < 1) return set_error(PUCK_ERR_ARG, "bad argument");
+    puck::Parser::Charts c;
+    h->p->debug_charts(words, offsets, n, layer, with_outside != 0, &c);
+    if ((int64_t)c.inside.size() > chart_cap) return set_error(PUCK_ERR_ARG, "chart buffer too small");
+    if (inside) std::memcpy(inside, c.inside.data(), c.inside.size() * 4);
+    if (outside) std::memcpy(outside, c.outside.data(), c.outside.size() * 4);
+    if (in_scale) std::memcpy(in_scale, c.in_scale.data(), c.in_scale.size() * 4);
+    if (out_scale) std::memcpy(out_scale, c.out_scale.data(), c.out_scale.size() * 4);
+    if (masks && !c.masks.empty()) std::memcpy(masks, c.masks.data(), c.masks.size() * 4);
+    if (partition_scales)
+      for (int32_t i = 0; i < n; ++i)
+        partition_scales[i] = i < (int32_t)c.partition_scales.size() ? c.partition_scales[i] : 0.0;
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_parser_stats(const puck_parser_t* h, puck_stats_t* out) {
+  if (!h || !out) return set_error(PUCK_ERR_ARG, "bad argument");
+  const puck::PassStats& s = h->p->stats;
+  out->rule_kernel_ms = s.rule_kernel_ms;
+  out->reduce_ms = s.reduce_ms;
+  out->other_ms = s.other_ms;
+  out->total_ms = s.total_ms;
+  out->rule_apps = s.rule_apps;
+  out->rule_launches = s.rule_launches;
+  return PUCK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// CKY engine: device tables for compiled grammar layers and the parse pipeline over batches.
+//
+// Host-side counterpart of CLParser / ActualParser (CLParser.scala:46-952) and WorkSpace
+// (WorkSpace.scala:15-123). The schedule (inside span loop, outside span loop, families per span,
+// coarse-to-fine fold over layers) follows CLParser.scala:50-67,256-369,489-522.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "grammar.h"
+
+namespace puck {
+
+// One node of a Viterbi tree, pre-order, 2n-1 nodes per sentence (CLViterbi.scala:378).
+struct TreeNode {
+  int32_t top_sym;  // nonterminal id of the unary top, -1 if the node has no unary layer
+  int32_t bot_sym;  // nonterminal id (width > 1) or terminal id (width == 1)
+  int32_t width;
+  float score;
+};
+
+// Per-cell MBR decode record (CLMBRKernels.scala:102: {score, symbol, _, _}).
+struct MbrCell {
+  float score;
+  int32_t symbol;  // coarse id
+};
+
+// One node of an MBR tree, pre-order: coarse labels, -1 top when the unary layer is skipped.
+struct MbrNode {
+  int32_t top_coarse;
+  int32_t bot_coarse;
+  int32_t begin, end;
+  float score;
+};
+
+enum ChartHalf : int32_t { kInBot = 0, kInTop = 1, kOutBot = 2, kOutTop = 3 };
+enum SplitKind : int32_t { kSkIn = 0, kSkOutL = 1, kSkOutR = 2, kSkUnary = 3 };
+
+// Family-directions of the rule passes (inside families, rotated outside families, unaries).
+enum FamilyId : int32_t {
+  kFInNN = 0, kFInNT, kFInTN, kFInTT,
+  kFOutLNN, kFOutLNT, kFOutLTN, kFOutLTT,
+  kFOutRNN, kFOutRNT, kFOutRTN, kFOutRTT,
+  kFInUN, kFInUT, kFOutUN, kFOutUT,
+  kNumFamilies
+};
+
+struct DevFamily {
+  int32_t id = 0;
+  int32_t kind = 0;          // SplitKind
+  int32_t range_split = 0;   // 1: split ranges over an interval, 0: one split
+  int32_t t_half = 0, a_half = 0, b_half = 0;  // ChartHalf of target and operands
+  int32_t n_t = 0, n_a = 0, n_b = 0;           // symbol space sizes
+  bool unary = false;
+  int32_t num_chunks = 0, num_slots = 0, num_rules = 0, max_rows = 0;
+  std::vector<int32_t> h_chunk_range;       // 4 per chunk: a_lo, a_hi, b_lo, b_hi
+  std::vector<int32_t> h_chunk_slot_start;  // num_chunks + 1
+  std::vector<int32_t> h_slot_target;
+  int32_t* d_chunk_range = nullptr;
+  int32_t* d_chunk_slot_start = nullptr;
+  int32_t* d_slot_target = nullptr;
+  int32_t* d_slot_rule_start = nullptr;
+  int32_t* d_slot_rule_count = nullptr;
+  uint4* d_rules = nullptr;  // {A row byte offset, B row byte offset, bits(q), 0}
+  // Block gating (CLParser.scala:783-785,845-855): block of each target symbol and block masks.
+  int32_t num_blocks = 0;
+  int32_t* d_block_of_target = nullptr;
+  uint32_t* d_block_masks = nullptr;
+  double rule_apps_per_item = 0;  // rules in the family (for roofline accounting)
+};
+
+// Reduction group: families whose partials are combined into one target chart half.
+struct DevReduce {
+  std::vector<int32_t> families;  // FamilyId list
+  int32_t n_t = 0;
+  int32_t* d_off = nullptr;   // n_t + 1
+  int32_t* d_fam = nullptr;   // index into families
+  int32_t* d_slot = nullptr;  // slot within that family
+};
+
+enum ReduceId : int32_t { kRInBin = 0, kRInBin2, kRInUN, kRInUT, kROutBin, kROutUN, kROutW1, kNumReduce };
+
+class Layer {
+ public:
+  Layer(Grammar g, bool final_layer, int32_t items_per_lane);
+  ~Layer();
+  Layer(const Layer&) = delete;
+  Layer& operator=(const Layer&) = delete;
+
+  Grammar g;
+  bool final_layer;
+  int32_t ipl;  // items per lane of the rule kernels (1 or 2)
+  DevFamily fam[kNumFamilies];
+  DevReduce red[kNumReduce];
+  // Decoder tables.
+  int32_t* d_dec_offsets = nullptr;
+  float* d_dec_scores = nullptr;
+  int32_t* d_dec_left = nullptr;
+  int32_t* d_dec_right = nullptr;
+  // Lexicon.
+  int32_t* d_lex_offsets = nullptr;
+  int32_t* d_lex_tags = nullptr;
+  double* d_lex_logp = nullptr;
+  int32_t* d_proj_n = nullptr;
+  int32_t* d_proj_t = nullptr;
+
+ private:
+  void build_family(int32_t id);
+  void build_reduce(int32_t rid, std::vector<int32_t> fams, int32_t n_t);
+  std::vector<void*> owned_;
+  template <class T> T* upload(const std::vector<T>& v);
+};
+
+// Timing and work counters of the last call (HIP events on the engine's stream).
+struct PassStats {
+  double rule_kernel_ms = 0;   // sum of rule-kernel durations
+  double reduce_ms = 0;
+  double other_ms = 0;
+  double rule_apps = 0;        // algorithmic rule applications executed by rule kernels
+  int64_t rule_launches = 0;
+  double total_ms = 0;
+  std::vector<float> launch_ms;      // per rule-kernel launch duration
+  std::vector<double> launch_apps;   // per rule-kernel launch rule applications
+  std::vector<int32_t> launch_family;
+};
+
+class Parser {
+ public:
+  Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
+         int32_t items_per_lane);
+  ~Parser();
+
+  // CLParser.parse: Viterbi trees (final layer Viterbi) or MBR trees (final layer Real).
+  void parse(const int32_t* words, const int32_t* offsets, int32_t n,
+             std::vector<std::vector<TreeNode>>* viterbi, std::vector<std::vector<MbrNode>>* mbr);
+  // CLParser.partitions: sentence inside log-likelihood under the final layer.
+  void partitions(const int32_t* words, const int32_t* offsets, int32_t n, float* out);
+  // CLParser.insideOutside: inside, outside and the decode kernels, results kept on device.
+  void inside_outside(const int32_t* words, const int32_t* offsets, int32_t n);
+  // Pruning masks of the last coarse layer for every cell of every sentence (maskWords per cell),
+  // plus its inside/outside scaling constants when the next layer is Real.
+  void masks(const int32_t* words, const int32_t* offsets, int32_t n, std::vector<uint32_t>* out,
+             std::vector<int32_t>* cell_offsets);
+
+  // Debug access for parity tests: run the named pipeline on one batch and copy back charts of
+  // layer `layer`: inside, outside (cells x cellSize), scales, masks of the coarse layer.
+  struct Charts {
+    std::vector<float> inside, outside, in_scale, out_scale;
+    std::vector<uint32_t> masks;
+    std::vector<int32_t> cell_offsets;
+    std::vector<double> partition_scales;
+    int32_t cell_size = 0;
+  };
+  void debug_charts(const int32_t* words, const int32_t* offsets, int32_t n, int32_t layer,
+                    bool outside, Charts* out);
+
+  PassStats stats;
+  const Layer& layer(int i) const { return *layers_[i]; }
+  int32_t num_layers() const { return (int32_t)layers_.size(); }
+  // Execution mode: 0 = one launch per span and family (the reference schedule), 1 = the same
+  // launches recorded once per batch shape into a HIP graph and replayed.
+  int32_t mode = 0;
+
+ private:
+  struct Batch;
+  std::vector<std::unique_ptr<Layer>> layers_;
+  int32_t device_;
+  int64_t mem_bytes_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  struct TimedLaunch { hipEvent_t a, b; double apps; int32_t family; };
+  std::vector<hipEvent_t> ev_pool_;
+  size_t ev_next_ = 0;
+  std::vector<TimedLaunch> pending_;
+  hipEvent_t next_event();
+  void flush_timing();
+
+  // Workspace (grown on demand).
+  struct Buf { void* p = nullptr; size_t bytes = 0; };
+  Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
+      pos_scale_, aux_;
+  void ensure(Buf& b, size_t bytes);
+
+  void run_batch(const int32_t* words, const int32_t* offsets, int32_t first, int32_t count,
+                 int32_t what, std::vector<std::vector<TreeNode>>* viterbi,
+                 std::vector<std::vector<MbrNode>>* mbr, float* partitions, Charts* charts,
+                 int32_t chart_layer, std::vector<uint32_t>* masks_out);
+  std::vector<std::pair<int32_t, int32_t>> make_batches(const int32_t* offsets, int32_t n);
+  void inside_pass(Batch& b, int32_t li, bool use_masks);
+  void outside_pass(Batch& b, int32_t li, bool use_masks);
+  void launch_family(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks,
+                     int32_t slice_cell0, int32_t slice_cells, float* partial_base);
+  void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks);
+  void timed_begin();
+  void timed_end(double* acc);
+};
+
+}  // namespace puck

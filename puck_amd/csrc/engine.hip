@@ -1,0 +1,1516 @@
+// CKY engine for gfx950: rule-family kernels, reductions, tag scores, masks, scaling, Viterbi and
+// MBR decode kernels, and the host schedule that drives them.
+//
+// Reference kernels replaced here (all OpenCL C generated as strings in the reference):
+//   binary/unary rule kernels  SimpleGenRuleMultiply.java:60-319
+//   gather/scatter             CLMatrixTransposeCopy.scala:123-220 (not needed: operands are staged
+//                              straight from the chart into LDS)
+//   computeMasks               CLMaskKernels.scala:154-238
+//   getScalingConstants        CLScalingKernels.scala:49-71
+//   viterbi                    CLViterbi.scala:364-608
+//   computeMBR                 CLMBRKernels.scala:101-173
+//   set/getRootScores          CLParserUtils.scala:66-77
+//   mem_zero                   ZeroMemoryKernel.scala:18-30
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace puck {
+
+#define HIP_CHECK(x)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess)                                                                    \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                               #x);                                                          \
+  } while (0)
+
+namespace {
+
+constexpr int kMaxLength = 255;  // cell lists pack the begin position in 8 bits
+
+__host__ __device__ inline int tri(int n) { return n * (n + 1) / 2; }
+
+// ChartHalf.chartIndex (ParseChart.scala:58-65): span-diagonal-major cell index.
+__host__ __device__ inline int chart_index(int b, int e, int n) {
+  int lo = b < e ? b : e;
+  int s = (b < e ? e - b : b - e) - 1;
+  return lo + s * n - s * (s - 1) / 2;
+}
+
+__device__ inline int cell_abs(const int32_t* cell_offsets, int sent, int n, int half, int b, int e) {
+  return cell_offsets[sent] + ((half & 1) ? tri(n) : 0) + chart_index(b, e, n);
+}
+
+__device__ inline bool mask_any(const uint32_t* m, int words) {
+  uint32_t x = 0;
+  for (int i = 0; i < words; ++i) x |= m[i];
+  return x != 0;
+}
+
+template <int SEMI>
+__device__ inline float semi_zero() {
+  return SEMI == kViterbi ? -INFINITY : 0.0f;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Rule-family kernel. One workgroup = one tile of ITEMS items (target cell, split) x one chunk
+// (row range of operand A x row range of operand B). Operand rows are staged item-minor in LDS so
+// that a wave reads one row for IPL items per lane with one ds_read. Waves pull target symbols
+// (slots) of the chunk from an LDS counter and stream that slot's rules (uniform across the wave)
+// with buffer loads. Each slot writes one partial value per item; the reduce kernel combines
+// partials over splits, chunks and families. Rule semantics per SimpleGenRuleMultiply.java:151
+// (parent = semiring_mad(parent, left (x) right, q)) and :213-216 (Real: *= scale).
+template <int IPL, int SEMI, bool UNARY>
+__global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
+  constexpr int ITEMS = 64 * IPL;
+  extern __shared__ __align__(16) float smem[];
+  __shared__ int32_t s_cell_a[ITEMS], s_cell_b[ITEMS];
+  __shared__ uint32_t s_gate[ITEMS];
+  __shared__ float s_scale[ITEMS];
+  __shared__ int32_t s_next;
+  const float zero = semi_zero<SEMI>();
+  const int tid = threadIdx.x;
+  const int tile0 = blockIdx.x * ITEMS;
+  const int chunk = blockIdx.y;
+  const int a_lo = a.chunk_range[4 * chunk], a_hi = a.chunk_range[4 * chunk + 1];
+  const int b_lo = a.chunk_range[4 * chunk + 2], b_hi = a.chunk_range[4 * chunk + 3];
+  const int rows_a = a_hi - a_lo;
+  const int rows_b = UNARY ? 0 : b_hi - b_lo;
+  float* lds_a = smem;
+  float* lds_b = smem + rows_a * a.row_stride;
+
+  if (tid == 0) s_next = 0;
+  if (tid < ITEMS) {
+    const int item = tile0 + tid;
+    int ca = 0, cb = 0;
+    uint32_t gate = 0;
+    float scale = 1.0f;
+    if (item < a.num_items) {
+      int k, j;
+      if (a.prefix == nullptr) {
+        k = item / a.uniform_count;
+        j = item - k * a.uniform_count;
+      } else {
+        const int abs_item = item + a.item_base;
+        int lo = 0, hi = a.num_cells;  // largest k with prefix[k] <= abs_item
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (a.prefix[mid] <= abs_item) lo = mid; else hi = mid;
+        }
+        k = lo;
+        j = abs_item - a.prefix[k];
+      }
+      const int packed = a.cell_list[k];
+      const int sent = packed >> 8, b = packed & 255, e = b + a.span;
+      const int n = a.lengths[sent];
+      int ax0 = b, ay0 = e, bx0 = 0, by0 = 0;
+      // Split ranges: CLParser.scala:345-350,492-518 (CLWorkQueueKernels.scala:138-168).
+      switch (a.kind) {
+        case 0: { int s = b + 1 + j; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;      // IN range
+        case 1: { int s = e - 1; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;          // IN e-1
+        case 2: { int s = b + 1; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;          // IN b+1
+        case 3: { int s = e + 1 + j; ax0 = b; ay0 = s; bx0 = e; by0 = s; } break;      // OUT_L range
+        case 4: { int s = e + 1; ax0 = b; ay0 = s; bx0 = e; by0 = s; } break;          // OUT_L e+1
+        case 5: { int s = j; ax0 = s; ay0 = b; bx0 = s; by0 = e; } break;              // OUT_R range
+        case 6: { int s = b - 1; ax0 = s; ay0 = b; bx0 = s; by0 = e; } break;          // OUT_R b-1
+        default: break;                                                                 // unary
+      }
+      ca = cell_abs(a.cell_offsets, sent, n, a.a_half, ax0, ay0);
+      cb = UNARY ? 0 : cell_abs(a.cell_offsets, sent, n, a.b_half, bx0, by0);
+      const int ct = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
+      if (a.masks == nullptr) {
+        gate = 0xffffffffu;
+      } else if (UNARY) {
+        // UnaryUpdateManager.doUpdates gates on the top mask of the cell (CLParser.scala:747).
+        const int ctop = cell_abs(a.cell_offsets, sent, n, 1, b, e);
+        gate = mask_any(a.masks + (size_t)ctop * a.mask_words, a.mask_words) ? 0xffffffffu : 0u;
+      } else {
+        // Children must be allowed spans (CLParser.scala:892-893) and the target cell's mask
+        // must intersect the block mask (CLParser.scala:845-855,867).
+        const bool ok_a = mask_any(a.masks + (size_t)ca * a.mask_words, a.mask_words);
+        const bool ok_b = mask_any(a.masks + (size_t)cb * a.mask_words, a.mask_words);
+        if (ok_a && ok_b) {
+          const uint32_t* tm = a.masks + (size_t)ct * a.mask_words;
+          for (int bb = 0; bb < a.num_blocks; ++bb) {
+            uint32_t hit = 0;
+            for (int w = 0; w < a.mask_words; ++w) hit |= tm[w] & a.block_masks[bb * a.mask_words + w];
+            if (hit) gate |= 1u << bb;
+          }
+        }
+      }
+      if (SEMI == kReal) {
+        // native_exp(-parentScale + rightScale + leftScale) (SimpleGenRuleMultiply.java:214,280)
+        scale = UNARY ? expf(-a.scT[ct] + a.scA[ca]) : expf(-a.scT[ct] + a.scB[cb] + a.scA[ca]);
+      }
+    }
+    s_cell_a[tid] = ca;
+    s_cell_b[tid] = cb;
+    s_gate[tid] = gate;
+    s_scale[tid] = scale;
+  }
+  __syncthreads();
+
+  // Stage operand rows: consecutive threads read consecutive symbols of one cell.
+  for (int idx = tid; idx < rows_a * ITEMS; idx += kBlockThreads) {
+    const int it = idx / rows_a;
+    const int row = idx - it * rows_a;
+    float v = zero;
+    if (tile0 + it < a.num_items) v = a.chA[(size_t)s_cell_a[it] * a.cell_size + a_lo + row];
+    lds_a[row * a.row_stride + it] = v;
+  }
+  if constexpr (!UNARY) {
+    for (int idx = tid; idx < rows_b * ITEMS; idx += kBlockThreads) {
+      const int it = idx / rows_b;
+      const int row = idx - it * rows_b;
+      float v = zero;
+      if (tile0 + it < a.num_items) v = a.chB[(size_t)s_cell_b[it] * a.cell_size + b_lo + row];
+      lds_b[row * a.row_stride + it] = v;
+    }
+  }
+  __syncthreads();
+
+  const int lane = tid & 63;
+  const int slot0 = a.chunk_slot_start[chunk];
+  const int nslots = a.chunk_slot_start[chunk + 1] - slot0;
+  const char* lane_base = reinterpret_cast<const char*>(smem) + lane * IPL * 4;
+  const int kz = (lane & a.kzero) * 16;  // zero at run time; keeps rule loads on the vector path
+  while (true) {
+    int s = 0;
+    if (lane == 0) s = atomicAdd(&s_next, 1);
+    s = __shfl(s, 0);
+    if (s >= nslots) break;
+    const int slot = slot0 + s;
+    const int t = a.slot_target[slot];
+    const int rs = a.slot_rule_start[slot];
+    const int rc = a.slot_rule_count[slot];
+    __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.rules + rs), (short)0, rc * 16, 0x00020000);
+    float acc[IPL];
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) acc[i] = zero;
+#pragma unroll 4
+    for (int r = 0; r < rc; ++r) {
+      const uint4 ru = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, kz, r * 16, 0));
+      const float q = __uint_as_float(ru.z);
+      if (IPL == 2) {
+        const float2 va = *reinterpret_cast<const float2*>(lane_base + ru.x);
+        if (UNARY) {
+          if (SEMI == kViterbi) {
+            acc[0] = fmaxf(acc[0], va.x + q);
+            acc[IPL - 1] = fmaxf(acc[IPL - 1], va.y + q);
+          } else {
+            acc[0] = fmaf(va.x, q, acc[0]);
+            acc[IPL - 1] = fmaf(va.y, q, acc[IPL - 1]);
+          }
+        } else {
+          const float2 vb = *reinterpret_cast<const float2*>(lane_base + ru.y);
+          if (SEMI == kViterbi) {
+            acc[0] = fmaxf(acc[0], (va.x + vb.x) + q);
+            acc[IPL - 1] = fmaxf(acc[IPL - 1], (va.y + vb.y) + q);
+          } else {
+            acc[0] = fmaf(va.x * vb.x, q, acc[0]);
+            acc[IPL - 1] = fmaf(va.y * vb.y, q, acc[IPL - 1]);
+          }
+        }
+      } else {
+        const float va = *reinterpret_cast<const float*>(lane_base + ru.x);
+        if (UNARY) {
+          acc[0] = SEMI == kViterbi ? fmaxf(acc[0], va + q) : fmaf(va, q, acc[0]);
+        } else {
+          const float vb = *reinterpret_cast<const float*>(lane_base + ru.y);
+          acc[0] = SEMI == kViterbi ? fmaxf(acc[0], (va + vb) + q) : fmaf(va * vb, q, acc[0]);
+        }
+      }
+    }
+    const int bit = (UNARY || a.block_of_target == nullptr) ? 0 : a.block_of_target[t];
+    float* out = a.partial + (size_t)slot * a.pitch;
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) {
+      const int li = lane * IPL + i;
+      const int item = tile0 + li;
+      float v = ((s_gate[li] >> bit) & 1u) ? acc[i] : zero;
+      if (SEMI == kReal) v *= s_scale[li];
+      if (item < a.num_items) out[item] = v;
+    }
+  }
+}
+
+// Combine partials: target[cell][t] = (+) over (family, slot) of t over the cell's items.
+template <int SEMI>
+__global__ void k_reduce(RedArgs a) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int t = blockIdx.y * 4 + threadIdx.y;
+  if (k >= a.num_cells || t >= a.n_t) return;
+  float acc = semi_zero<SEMI>();
+  for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
+    const int f = a.fam[e];
+    const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
+    int j0, j1;
+    if (a.prefix[f] == nullptr) {
+      j0 = k * a.uniform_count[f];
+      j1 = j0 + a.uniform_count[f];
+    } else {
+      j0 = a.prefix[f][k] - a.base[f];
+      j1 = a.prefix[f][k + 1] - a.base[f];
+    }
+    for (int j = j0; j < j1; ++j) {
+      if (SEMI == kViterbi) acc = fmaxf(acc, p[j]); else acc += p[j];
+    }
+  }
+  const int packed = a.cell_list[k];
+  const int sent = packed >> 8, b = packed & 255, e = b + a.span;
+  const int n = a.lengths[sent];
+  if (a.root_init && b == 0 && e == n) {
+    // setRootScores (CLParserUtils.scala:66-70) on the outside top root cell.
+    acc = (t == a.root) ? a.one : semi_zero<SEMI>();
+  }
+  const int c = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
+  a.chT[(size_t)c * a.cell_size + t] = acc;
+}
+
+__global__ void k_fill(float* p, size_t n, float v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Span-1 tag scores (CLParser.scala:411-467): one thread per word position.
+template <int SEMI>
+__global__ void k_tags(float* inside, const int32_t* cell_offsets, const int32_t* lengths,
+                       const int32_t* len_offsets, const int32_t* words, int32_t num_sent,
+                       int32_t total_pos, const int32_t* lex_off, const int32_t* lex_tag,
+                       const double* lex_logp, const float* in_scale, double* pos_scale,
+                       int32_t cell_size) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total_pos) return;
+  int lo = 0, hi = num_sent;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (len_offsets[mid] <= p) lo = mid; else hi = mid;
+  }
+  const int sent = lo, pos = p - len_offsets[sent], n = lengths[sent];
+  const int cell = cell_offsets[sent] + chart_index(pos, pos + 1, n);
+  float* dst = inside + (size_t)cell * cell_size;
+  const int w = words[p];
+  if (SEMI == kViterbi) {
+    for (int e = lex_off[w]; e < lex_off[w + 1]; ++e) dst[lex_tag[e]] = (float)lex_logp[e];
+  } else {
+    const float my_scale = (float)exp(-(double)in_scale[cell]);
+    float sum = 0.0f;
+    for (int e = lex_off[w]; e < lex_off[w + 1]; ++e) {
+      const float v = (float)exp((double)(float)lex_logp[e]) * my_scale;
+      dst[lex_tag[e]] = v;
+      sum = sum + v;
+    }
+    const float half = sum / 2;
+    for (int e = lex_off[w]; e < lex_off[w + 1]; ++e) dst[lex_tag[e]] = dst[lex_tag[e]] / half;
+    pos_scale[p] = log((double)half);
+  }
+}
+
+// computeMasks (CLMaskKernels.scala:172-236): one wave per cell, bits OR-reduced across the wave.
+__global__ void k_masks(uint32_t* masks_out, const float* inside, const float* outside,
+                        const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
+                        int32_t num_cells, const int32_t* proj_t, const int32_t* proj_n,
+                        int32_t num_syms, int32_t cell_size, int32_t root, float thresh,
+                        int32_t mask_words) {
+  const int cell = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (cell >= num_cells) return;
+  int lo = 0, hi = num_sent;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (cell_offsets[mid] <= cell) lo = mid; else hi = mid;
+  }
+  const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
+  const int n = lengths[sent];
+  const float root_score = inside[(size_t)(last - 1) * cell_size + root];
+  const float cutoff = root_score + thresh;
+  const bool term = (cell - first) < n;
+  const int32_t* proj = term ? proj_t : proj_n;
+  const int nproj = term ? 0 : 0;
+  (void)nproj;
+  const float* in = inside + (size_t)cell * cell_size;
+  const float* out = outside + (size_t)cell * cell_size;
+  uint32_t m[4] = {0, 0, 0, 0};
+  for (int sym = lane; sym < num_syms; sym += 64) {
+    const float score = in[sym] + out[sym];
+    if (score >= cutoff) {
+      const int field = proj[sym];
+      m[field >> 5] |= 1u << (field & 31);
+    }
+  }
+  for (int w = 0; w < mask_words; ++w) {
+    uint32_t v = m[w];
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+    if (lane == 0) masks_out[(size_t)cell * mask_words + w] = v;
+  }
+}
+
+// getScalingConstants (CLScalingKernels.scala:51-69).
+__global__ void k_scaling(float* scale, const float* chart, int32_t num_syms, int32_t num_cells) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= num_cells) return;
+  float m = -100000.0f;
+  const float* c = chart + (size_t)cell * num_syms;
+  for (int sym = 0; sym < num_syms; ++sym) m = fmaxf(c[sym], m);
+  if (m < -9000.0f) m = 0;
+  scale[cell] = m;
+}
+
+// computeMBR (CLMBRKernels.scala:118-171): one thread per cell, symbols in order.
+__global__ void k_mbr(MbrCell* out, const float* inside, const float* outside,
+                      const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
+                      int32_t num_cells, const int32_t* proj_t, const int32_t* proj_n,
+                      int32_t num_syms, int32_t cell_size, int32_t root, int32_t num_coarse) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= num_cells) return;
+  int lo = 0, hi = num_sent;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (cell_offsets[mid] <= cell) lo = mid; else hi = mid;
+  }
+  const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
+  const int n = lengths[sent];
+  const float root_score = inside[(size_t)(last - 1) * cell_size + root];
+  const int32_t* proj = (cell - first) < n ? proj_t : proj_n;
+  const float* in = inside + (size_t)cell * cell_size;
+  const float* o = outside + (size_t)cell * cell_size;
+  float margs[128];
+  for (int c = 0; c < num_coarse; ++c) margs[c] = 0.0f;
+  for (int sym = 0; sym < num_syms; ++sym) {
+    const float v = in[sym] / root_score * o[sym];
+    margs[proj[sym]] += v;
+  }
+  MbrCell d{-INFINITY, -1};
+  for (int c = 0; c < num_coarse; ++c) {
+    if (margs[c] > d.score) {
+      d.score = margs[c];
+      d.symbol = c;
+    }
+  }
+  out[cell] = d;
+}
+
+// viterbi (CLViterbi.scala:495-605): top-down best tree, one wave per sentence. The reference scans
+// candidates sequentially with '>=' (the last best candidate wins); here lanes scan strided subsets
+// keeping their last best, and the wave picks the best score with ties going to the larger
+// candidate index, which selects the same candidate.
+struct Best {
+  float s;
+  int i;
+};
+
+__device__ inline Best wave_best(Best b) {
+  for (int o = 32; o > 0; o >>= 1) {
+    float os = __shfl_xor(b.s, o);
+    int oi = __shfl_xor(b.i, o);
+    if (os > b.s || (os == b.s && oi > b.i)) {
+      b.s = os;
+      b.i = oi;
+    }
+  }
+  return b;
+}
+
+__global__ void k_viterbi(TreeNode* tree_out, const int32_t* tree_offsets, const float* inside,
+                          const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
+                          int32_t cell_size, int32_t root, int32_t nn, const int32_t* offsets,
+                          const float* scores, const int32_t* lefts, const int32_t* rights) {
+  __shared__ int32_t s_top[2 * kMaxLength], s_bot[2 * kMaxLength], s_width[2 * kMaxLength];
+  __shared__ float s_score[2 * kMaxLength];
+  const int sent = blockIdx.x;
+  if (sent >= num_sent) return;
+  const int lane = threadIdx.x;
+  const int n = lengths[sent];
+  const float* ibot = inside + (size_t)cell_offsets[sent] * cell_size;
+  const float* itop = ibot + (size_t)tri(n) * cell_size;
+  const int nodes = 2 * n - 1;
+  for (int p = lane; p < nodes; p += 64) {
+    s_top[p] = 0;
+    s_bot[p] = 0;
+    s_width[p] = 0;
+    s_score[p] = 0.0f;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s_top[0] = root;
+    s_width[0] = n;
+  }
+  __syncthreads();
+  const int g_nn = 0, g_nt = nn, g_tn = 2 * nn, g_tt = 3 * nn, g_u = 4 * nn, g_ut = 5 * nn;
+  int begin = 0;
+  for (int p = 0; p < nodes; ++p) {
+    const int width = s_width[p];
+    const int end = begin + width;
+    int bot = s_bot[p];
+    if (s_top[p] != -1) {
+      // BestUnary (CLViterbi.scala:383-425): init -30000, sym 0.
+      const int g = (width == 1) ? g_ut : g_u;
+      const int r0 = offsets[g + s_top[p]], r1 = offsets[g + s_top[p] + 1];
+      const float* cell = ibot + (size_t)chart_index(begin, end, n) * cell_size;
+      Best b{-30000.0f, -1};
+      for (int r = r0 + lane; r < r1; r += 64) {
+        const float sc = scores[r] + cell[lefts[r]];
+        if (sc >= b.s) {
+          b.s = sc;
+          b.i = r;
+        }
+      }
+      b = wave_best(b);
+      bot = b.i >= 0 ? lefts[b.i] : 0;
+      if (lane == 0) {
+        s_bot[p] = bot;
+        s_score[p] = b.s;
+      }
+    }
+    if (width == 1) {
+      if (lane == 0) s_score[p] = ibot[(size_t)cell_size * begin + bot];
+      begin += 1;
+    } else {
+      const int nt0 = offsets[g_nt + bot], nt1 = offsets[g_nt + bot + 1];
+      const int tn0 = offsets[g_tn + bot], tn1 = offsets[g_tn + bot + 1];
+      const int tt0 = offsets[g_tt + bot], tt1 = offsets[g_tt + bot + 1];
+      const int nn0 = offsets[g_nn + bot], nn1 = offsets[g_nn + bot + 1];
+      const int c0 = nt1 - nt0;
+      const int c1 = c0 + (tn1 - tn0);
+      const int c2 = c1 + (width == 2 ? tt1 - tt0 : 0);
+      const int nnn = nn1 - nn0;
+      const int total = c2 + nnn * (width - 1);
+      const float* nt_l = itop + (size_t)chart_index(begin, end - 1, n) * cell_size;
+      const float* nt_r = ibot + (size_t)chart_index(end - 1, end, n) * cell_size;
+      const float* tn_l = ibot + (size_t)chart_index(begin, begin + 1, n) * cell_size;
+      const float* tn_r = itop + (size_t)chart_index(begin + 1, end, n) * cell_size;
+      const float* tt_r = ibot + (size_t)chart_index(begin + 1, end, n) * cell_size;
+      Best b{-300000.0f, -1};
+      for (int c = lane; c < total; c += 64) {
+        float sc;
+        if (c < c0) {
+          const int r = nt0 + c;
+          sc = (scores[r] + nt_l[lefts[r]]) + nt_r[rights[r]];
+        } else if (c < c1) {
+          const int r = tn0 + (c - c0);
+          sc = (scores[r] + tn_l[lefts[r]]) + tn_r[rights[r]];
+        } else if (c < c2) {
+          const int r = tt0 + (c - c1);
+          sc = (scores[r] + tn_l[lefts[r]]) + tt_r[rights[r]];
+        } else {
+          const int cc = c - c2;
+          const int si = cc / nnn;
+          const int r = nn0 + (cc - si * nnn);
+          const int split = begin + 1 + si;
+          const float* l = itop + (size_t)chart_index(begin, split, n) * cell_size;
+          const float* rr = itop + (size_t)chart_index(split, end, n) * cell_size;
+          sc = (scores[r] + l[lefts[r]]) + rr[rights[r]];
+        }
+        if (sc >= b.s) {
+          b.s = sc;
+          b.i = c;
+        }
+      }
+      b = wave_best(b);
+      if (lane == 0) {
+        int config, best_split, left = 10, right = 10;
+        if (b.i < 0) {
+          config = 0;
+          best_split = begin + 1;
+        } else if (b.i < c0) {
+          config = 1;
+          best_split = end - 1;
+          left = lefts[nt0 + b.i];
+          right = rights[nt0 + b.i];
+        } else if (b.i < c1) {
+          config = 2;
+          best_split = begin + 1;
+          left = lefts[tn0 + b.i - c0];
+          right = rights[tn0 + b.i - c0];
+        } else if (b.i < c2) {
+          config = 3;
+          best_split = end - 1;
+          left = lefts[tt0 + b.i - c1];
+          right = rights[tt0 + b.i - c1];
+        } else {
+          const int cc = b.i - c2;
+          const int si = cc / nnn;
+          config = 0;
+          best_split = begin + 1 + si;
+          left = lefts[nn0 + cc - si * nnn];
+          right = rights[nn0 + cc - si * nnn];
+        }
+        s_score[p] = b.s;
+        const int lc = p + 1, rc = p + 2 * (best_split - begin);
+        if ((config >> 1) == 0) {
+          s_top[lc] = left;
+        } else {
+          s_top[lc] = -1;
+          s_bot[lc] = left;
+        }
+        if ((config & 1) == 0) {
+          s_top[rc] = right;
+        } else {
+          s_top[rc] = -1;
+          s_bot[rc] = right;
+        }
+        s_width[lc] = best_split - begin;
+        s_width[rc] = end - best_split;
+      }
+    }
+    __syncthreads();
+  }
+  TreeNode* out = tree_out + tree_offsets[sent];
+  for (int p = lane; p < nodes; p += 64) out[p] = TreeNode{s_top[p], s_bot[p], s_width[p], s_score[p]};
+}
+
+__global__ void k_get_root(float* out, const float* inside, const int32_t* cell_offsets,
+                           int32_t num_sent, int32_t cell_size, int32_t root) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_sent) return;
+  out[i] = inside[(size_t)(cell_offsets[i + 1] - 1) * cell_size + root];
+}
+
+template <class F>
+void launch_semi(int semi, F&& f) {
+  if (semi == kViterbi) f(std::integral_constant<int, kViterbi>{});
+  else if (semi == kReal) f(std::integral_constant<int, kReal>{});
+  else throw std::runtime_error("semiring not supported by the device engine");
+}
+
+}  // namespace
+
+// ================================================================================================
+// Layer: device tables for one compiled grammar.
+
+template <class T>
+T* Layer::upload(const std::vector<T>& v) {
+  T* p = nullptr;
+  const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+  HIP_CHECK(hipMalloc(&p, bytes));
+  if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  owned_.push_back(p);
+  return p;
+}
+
+Layer::Layer(Grammar gr, bool final_l, int32_t items_per_lane)
+    : g(std::move(gr)), final_layer(final_l), ipl(items_per_lane) {
+  if (ipl != 1 && ipl != 2) throw std::runtime_error("items per lane must be 1 or 2");
+  if (g.mask_words > 4) throw std::runtime_error("at most 128 coarse symbols are supported");
+  for (int f = 0; f < kNumFamilies; ++f) build_family(f);
+  build_reduce(kRInBin, {kFInNN, kFInNT, kFInTN}, g.num_nonterms());
+  build_reduce(kRInBin2, {kFInNN, kFInNT, kFInTN, kFInTT}, g.num_nonterms());
+  build_reduce(kRInUN, {kFInUN}, g.num_nonterms());
+  build_reduce(kRInUT, {kFInUT}, g.num_nonterms());
+  build_reduce(kROutBin, {kFOutRTN, kFOutLNT, kFOutLNN, kFOutRNN}, g.num_nonterms());
+  build_reduce(kROutUN, {kFOutUN}, g.num_nonterms());
+  build_reduce(kROutW1, {kFOutUT, kFOutLTT, kFOutRNT, kFOutLTN, kFOutRTT}, g.num_terms());
+  d_dec_offsets = upload(g.dec_offsets);
+  d_dec_scores = upload(g.dec_scores);
+  d_dec_left = upload(g.dec_left);
+  d_dec_right = upload(g.dec_right);
+  d_lex_offsets = upload(g.lex_offsets);
+  d_lex_tags = upload(g.lex_tags);
+  d_lex_logp = upload(g.lex_logp);
+  std::vector<int32_t> pn = g.proj_n, pt = g.proj_t;
+  pn.resize(g.num_syms, 0);
+  pt.resize(g.num_syms, 0);
+  d_proj_n = upload(pn);
+  d_proj_t = upload(pt);
+}
+
+Layer::~Layer() {
+  for (void* p : owned_) (void)hipFree(p);
+}
+
+void Layer::build_family(int32_t id) {
+  DevFamily& F = fam[id];
+  F.id = id;
+  const int nN = g.num_nonterms(), nT = g.num_terms();
+  struct R4 { int32_t t, a, b; float q; };
+  std::vector<R4> rules;
+  const BlockPartition* bp = nullptr;
+  bool t_term = false;
+  auto binfam = [&](int f) { return g.binaries[f]; };
+  if (id <= kFInTT) {
+    const int f = id - kFInNN;
+    for (auto& r : binfam(f)) rules.push_back({r.p, r.l, r.r, r.q});
+    F.kind = (f == kNN) ? 0 : (f == kNT ? 1 : 2);
+    F.range_split = f == kNN;
+    F.t_half = kInBot;
+    F.a_half = (f == kNN || f == kNT) ? kInTop : kInBot;
+    F.b_half = (f == kNN || f == kTN) ? kInTop : kInBot;
+    F.n_t = nN;
+    F.n_a = (f == kTN || f == kTT) ? nT : nN;
+    F.n_b = (f == kNT || f == kTT) ? nT : nN;
+    bp = &g.inside_blocks[f];
+  } else if (id <= kFOutLTT) {
+    const int f = id - kFOutLNN;  // rotateLeftToParent: target = left child, A = parent, B = right
+    for (auto& r : binfam(f)) rules.push_back({r.l, r.p, r.r, r.q});
+    const bool range = (f == kNN || f == kTN);
+    F.kind = range ? 3 : 4;
+    F.range_split = range;
+    t_term = (f == kTN || f == kTT);
+    F.t_half = t_term ? kOutBot : kOutTop;
+    F.a_half = kOutBot;
+    F.b_half = (f == kNN || f == kTN) ? kInTop : kInBot;
+    F.n_t = t_term ? nT : nN;
+    F.n_a = nN;
+    F.n_b = (f == kNT || f == kTT) ? nT : nN;
+    bp = &g.outside_l_blocks[f];
+  } else if (id <= kFOutRTT) {
+    const int f = id - kFOutRNN;  // rotateRightToParent: target = right child, A = left, B = parent
+    for (auto& r : binfam(f)) rules.push_back({r.r, r.l, r.p, r.q});
+    const bool range = (f == kNN || f == kNT);
+    F.kind = range ? 5 : 6;
+    F.range_split = range;
+    t_term = (f == kNT || f == kTT);
+    F.t_half = t_term ? kOutBot : kOutTop;
+    F.a_half = (f == kNN || f == kNT) ? kInTop : kInBot;
+    F.b_half = kOutBot;
+    F.n_t = t_term ? nT : nN;
+    F.n_a = (f == kTN || f == kTT) ? nT : nN;
+    F.n_b = nN;
+    bp = &g.outside_r_blocks[f];
+  } else {
+    F.unary = true;
+    F.kind = 7;
+    F.range_split = 0;
+    if (id == kFInUN || id == kFInUT) {
+      const auto& us = id == kFInUN ? g.unary_n : g.unary_t;
+      for (auto& u : us) rules.push_back({u.p, u.c, 0, u.q});
+      F.t_half = kInTop;
+      F.a_half = kInBot;
+      F.n_t = nN;
+      F.n_a = id == kFInUN ? nN : nT;
+    } else {
+      const auto& us = id == kFOutUN ? g.unary_n : g.unary_t;
+      for (auto& u : us) rules.push_back({u.c, u.p, 0, u.q});  // rotateChildToParent
+      t_term = id == kFOutUT;
+      F.t_half = kOutBot;
+      F.a_half = kOutTop;
+      F.n_t = t_term ? nT : nN;
+      F.n_a = nN;
+    }
+    F.n_b = 0;
+    F.b_half = 0;
+  }
+  F.num_rules = (int32_t)rules.size();
+  F.rule_apps_per_item = (double)rules.size();
+
+  // Chunking of the operand symbol spaces so that one tile's rows fit the LDS budget.
+  const int stride = 64 * ipl + (ipl == 1 ? 1 : 2);
+  const int max_rows = kLdsBudgetBytes / (stride * 4);
+  int ca = 1, cb = 1;
+  if (F.unary) {
+    ca = (F.n_a + max_rows - 1) / max_rows;
+  } else {
+    int best = 1 << 30;
+    for (int x = 1; x <= 64; ++x)
+      for (int y = 1; y <= 64; ++y) {
+        int ra = (F.n_a + x - 1) / x, rb = (F.n_b + y - 1) / y;
+        if (ra + rb <= max_rows && x * y < best) {
+          best = x * y;
+          ca = x;
+          cb = y;
+        }
+      }
+  }
+  ca = std::max(1, ca);
+  const int ra = (std::max(1, F.n_a) + ca - 1) / ca;
+  const int rb = F.unary ? 0 : (std::max(1, F.n_b) + cb - 1) / cb;
+  F.max_rows = ra + rb;
+  std::vector<int32_t> slot_rule_start, slot_rule_count;
+  std::vector<uint4> packed;
+  F.h_chunk_slot_start.push_back(0);
+  for (int ia = 0; ia < ca; ++ia)
+    for (int ib = 0; ib < (F.unary ? 1 : cb); ++ib) {
+      const int a_lo = ia * ra, a_hi = std::min(F.n_a, a_lo + ra);
+      const int b_lo = F.unary ? 0 : ib * rb, b_hi = F.unary ? 0 : std::min(F.n_b, b_lo + rb);
+      std::vector<R4> cr;
+      for (auto& r : rules)
+        if (r.a >= a_lo && r.a < a_hi && (F.unary || (r.b >= b_lo && r.b < b_hi))) cr.push_back(r);
+      std::stable_sort(cr.begin(), cr.end(), [](const R4& x, const R4& y) {
+        if (x.t != y.t) return x.t < y.t;
+        if (x.a != y.a) return x.a < y.a;
+        return x.b < y.b;
+      });
+      F.h_chunk_range.insert(F.h_chunk_range.end(), {a_lo, a_hi, b_lo, b_hi});
+      const int rows_a = a_hi - a_lo;
+      for (size_t i = 0; i < cr.size();) {
+        size_t j = i;
+        while (j < cr.size() && cr[j].t == cr[i].t) ++j;
+        F.h_slot_target.push_back(cr[i].t);
+        slot_rule_start.push_back((int32_t)packed.size());
+        slot_rule_count.push_back((int32_t)(j - i));
+        for (size_t k = i; k < j; ++k) {
+          uint4 u;
+          u.x = (uint32_t)((cr[k].a - a_lo) * stride * 4);
+          u.y = F.unary ? 0u : (uint32_t)((rows_a + cr[k].b - b_lo) * stride * 4);
+          std::memcpy(&u.z, &cr[k].q, 4);
+          u.w = 0;
+          packed.push_back(u);
+        }
+        i = j;
+      }
+      F.h_chunk_slot_start.push_back((int32_t)F.h_slot_target.size());
+      ++F.num_chunks;
+    }
+  F.num_slots = (int32_t)F.h_slot_target.size();
+  F.d_chunk_range = upload(F.h_chunk_range);
+  F.d_chunk_slot_start = upload(F.h_chunk_slot_start);
+  F.d_slot_target = upload(F.h_slot_target);
+  F.d_slot_rule_start = upload(slot_rule_start);
+  F.d_slot_rule_count = upload(slot_rule_count);
+  F.d_rules = upload(packed);
+
+  if (!F.unary) {
+    // Block gating tables. Non-final layers merge all kernels into one block whose mask is the
+    // first kernel's (CLParser.scala:783-785).
+    std::vector<int32_t> bot(F.n_t, 0);
+    std::vector<uint32_t> masks;
+    if (final_layer) {
+      F.num_blocks = bp->num_blocks;
+      masks = bp->masks;
+      for (int t = 0; t < F.n_t; ++t) {
+        const int c = t_term ? g.proj_t[t] : g.proj_n[t];
+        bot[t] = std::max(0, bp->block_of_coarse[c]);
+      }
+    } else {
+      F.num_blocks = bp->num_blocks > 0 ? 1 : 0;
+      masks.assign(bp->masks.begin(), bp->masks.begin() + std::min<size_t>(bp->masks.size(), g.mask_words));
+    }
+    if (F.num_blocks > 32) throw std::runtime_error("more than 32 rule blocks in one family");
+    masks.resize(std::max<size_t>(masks.size(), 1), 0u);
+    F.d_block_of_target = upload(bot);
+    F.d_block_masks = upload(masks);
+  }
+}
+
+void Layer::build_reduce(int32_t rid, std::vector<int32_t> fams, int32_t n_t) {
+  DevReduce& R = red[rid];
+  R.families = fams;
+  R.n_t = n_t;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> per(n_t);
+  for (size_t fi = 0; fi < fams.size(); ++fi) {
+    const DevFamily& F = fam[fams[fi]];
+    if (F.n_t != n_t) throw std::runtime_error("reduce group mixes target spaces");
+    for (int s = 0; s < F.num_slots; ++s) per[F.h_slot_target[s]].push_back({(int32_t)fi, s});
+  }
+  std::vector<int32_t> off(1, 0), fv, sv;
+  for (auto& v : per) {
+    for (auto& pr : v) {
+      fv.push_back(pr.first);
+      sv.push_back(pr.second);
+    }
+    off.push_back((int32_t)fv.size());
+  }
+  R.d_off = upload(off);
+  R.d_fam = upload(fv);
+  R.d_slot = upload(sv);
+}
+
+// ================================================================================================
+// Parser
+
+struct Parser::Batch {
+  int32_t n = 0, max_len = 0, num_cells = 0, total_len = 0;
+  std::vector<int32_t> lengths, cell_offsets, len_offsets, words, tree_offsets;
+  int32_t *d_lengths = nullptr, *d_cell_offsets = nullptr, *d_len_offsets = nullptr,
+          *d_words = nullptr, *d_tree_offsets = nullptr, *d_cells = nullptr;
+  std::vector<int32_t> span_start, span_count;  // cell list per span (index w)
+  std::vector<int32_t> h_cells;
+  // Item prefixes of variable-count families: (span, family) -> offset into d_prefix, -1 if none.
+  std::vector<int64_t> prefix_off;
+  int32_t* d_prefix = nullptr;
+  std::vector<int32_t> h_prefix;
+  std::vector<double> partition_scales;
+  int32_t mask_in = 0;   // which half of masks_ holds the current input masks
+  bool has_masks = false;
+  bool has_scales = false;
+};
+
+Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
+               int32_t items_per_lane)
+    : device_(device), mem_bytes_(mem_bytes) {
+  if (layers.empty()) throw std::runtime_error("parser needs at least one grammar layer");
+  HIP_CHECK(hipSetDevice(device_));
+  for (size_t i = 0; i < layers.size(); ++i) {
+    if (i + 1 < layers.size() && layers[i]->semiring != kViterbi)
+      throw std::runtime_error("coarse layers must use the Viterbi semiring");
+    if (layers[i]->mask_words != layers[0]->mask_words ||
+        layers[i]->coarse_names != layers[0]->coarse_names)
+      throw std::runtime_error("all layers must share the coarse symbol set");
+    layers_.emplace_back(new Layer(*layers[i], i + 1 == layers.size(), items_per_lane));
+  }
+  HIP_CHECK(hipStreamCreate(&stream_));
+  HIP_CHECK(hipEventCreate(&ev0_));
+  HIP_CHECK(hipEventCreate(&ev1_));
+  // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
+  const int lds = kLdsBudgetBytes;
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kViterbi, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kViterbi, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kReal, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kReal, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kViterbi, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kViterbi, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kReal, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kReal, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+}
+
+Parser::~Parser() {
+  for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
+                 &trees_, &mbr_, &pos_scale_, &aux_})
+    if (b->p) (void)hipFree(b->p);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Parser::ensure(Buf& b, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 256);
+  if (b.bytes >= bytes) return;
+  if (b.p) HIP_CHECK(hipFree(b.p));
+  b.p = nullptr;
+  b.bytes = 0;
+  HIP_CHECK(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+}
+
+// WorkSpace.getBatches (WorkSpace.scala:20-41): close a batch when its chart cells exceed the
+// chart budget.
+std::vector<std::pair<int32_t, int32_t>> Parser::make_batches(const int32_t* offsets, int32_t n) {
+  int32_t max_cell = 0;
+  for (auto& l : layers_) max_cell = std::max(max_cell, l->g.cell_size);
+  // Two charts per layer pass plus the rule partial buffer; keep 60% of the budget for charts.
+  const int64_t chart_budget = (int64_t)(mem_bytes_ * 0.6) / 2;
+  const int64_t max_cells = std::max<int64_t>(1, chart_budget / ((int64_t)max_cell * 4));
+  std::vector<std::pair<int32_t, int32_t>> out;
+  int32_t start = 0;
+  int64_t cells = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t len = offsets[i + 1] - offsets[i];
+    const int64_t c = (int64_t)len * (len + 1);
+    if (i > start && cells + c > max_cells) {
+      out.push_back({start, i - start});
+      start = i;
+      cells = 0;
+    }
+    cells += c;
+  }
+  if (n > start) out.push_back({start, n - start});
+  return out;
+}
+
+void Parser::timed_begin() { HIP_CHECK(hipEventRecord(ev0_, stream_)); }
+void Parser::timed_end(double* acc) {
+  HIP_CHECK(hipEventRecord(ev1_, stream_));
+  HIP_CHECK(hipEventSynchronize(ev1_));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  *acc += ms;
+}
+
+namespace {
+// Items a family generates at cell (b, e) of a sentence of length n.
+int items_for(int kind, int b, int e, int n) {
+  switch (kind) {
+    case 0: return e - b - 1;
+    case 1: case 2: return 1;
+    case 3: return n - e;
+    case 4: return e + 1 <= n ? 1 : 0;
+    case 5: return b;
+    case 6: return b >= 1 ? 1 : 0;
+    default: return 1;
+  }
+}
+bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
+}  // namespace
+
+void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks,
+                           int32_t k0, int32_t kc, float* partial) {
+  Layer& L = *layers_[li];
+  const DevFamily& F = L.fam[fid];
+  const Grammar& g = L.g;
+  FamArgs a{};
+  float* inside = (float*)inside_.p;
+  float* outside = (float*)outside_.p;
+  a.chA = F.a_half >= 2 ? outside : inside;
+  a.chB = F.b_half >= 2 ? outside : inside;
+  float* isc = (float*)in_scale_.p;
+  float* osc = (float*)out_scale_.p;
+  a.scT = F.t_half >= 2 ? osc : isc;
+  a.scA = F.a_half >= 2 ? osc : isc;
+  a.scB = F.b_half >= 2 ? osc : isc;
+  a.masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words : nullptr;
+  a.cell_offsets = B.d_cell_offsets;
+  a.lengths = B.d_lengths;
+  a.cell_list = B.d_cells + B.span_start[span] + k0;
+  a.num_cells = kc;
+  int64_t nitems;
+  if (uniform_kind(F.kind)) {
+    a.prefix = nullptr;
+    a.uniform_count = F.kind == 0 ? span - 1 : 1;
+    nitems = (int64_t)kc * a.uniform_count;
+    a.item_base = 0;
+  } else {
+    const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+    a.prefix = B.d_prefix + po + k0;
+    a.item_base = B.h_prefix[po + k0];
+    nitems = B.h_prefix[po + k0 + kc] - a.item_base;
+    a.uniform_count = 1;
+  }
+  if (nitems <= 0) return;
+  a.num_items = (int32_t)nitems;
+  a.chunk_range = F.d_chunk_range;
+  a.chunk_slot_start = F.d_chunk_slot_start;
+  a.slot_target = F.d_slot_target;
+  a.slot_rule_start = F.d_slot_rule_start;
+  a.slot_rule_count = F.d_slot_rule_count;
+  a.rules = F.d_rules;
+  a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
+  a.block_masks = F.unary ? nullptr : F.d_block_masks;
+  a.num_blocks = F.num_blocks;
+  a.partial = partial;
+  const int items_tile = 64 * L.ipl;
+  a.pitch = (int32_t)((nitems + 255) / 256 * 256);
+  a.kzero = 0;
+  a.span = span;
+  a.kind = F.kind;
+  a.range_split = F.range_split;
+  a.cell_size = g.cell_size;
+  a.a_half = F.a_half;
+  a.b_half = F.b_half;
+  a.t_half = F.t_half;
+  a.mask_words = g.mask_words;
+  a.row_stride = 64 * L.ipl + (L.ipl == 1 ? 1 : 2);
+  dim3 grid((unsigned)((nitems + items_tile - 1) / items_tile), (unsigned)F.num_chunks);
+  const size_t lds = (size_t)F.max_rows * a.row_stride * 4;
+  hipEvent_t t0 = next_event(), t1 = next_event();
+  HIP_CHECK(hipEventRecord(t0, stream_));
+  launch_semi(g.semiring, [&](auto semi) {
+    constexpr int S = decltype(semi)::value;
+    if (L.ipl == 2) {
+      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<2, S, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+    } else {
+      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<1, S, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+    }
+  });
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(t1, stream_));
+  const double apps = (double)nitems * F.rule_apps_per_item;
+  pending_.push_back(TimedLaunch{t0, t1, apps, fid});
+  stats.rule_apps += apps;
+  stats.rule_launches += 1;
+}
+
+hipEvent_t Parser::next_event() {
+  if (ev_next_ == ev_pool_.size()) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+  }
+  return ev_pool_[ev_next_++];
+}
+
+void Parser::flush_timing() {
+  HIP_CHECK(hipStreamSynchronize(stream_));
+  for (const TimedLaunch& t : pending_) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+    stats.rule_kernel_ms += ms;
+    stats.launch_ms.push_back(ms);
+    stats.launch_apps.push_back(t.apps);
+    stats.launch_family.push_back(t.family);
+  }
+  pending_.clear();
+  ev_next_ = 0;
+}
+
+void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use_masks) {
+  Layer& L = *layers_[li];
+  const DevReduce& R = L.red[rid];
+  const Grammar& g = L.g;
+  const int32_t ncells = B.span_count[span];
+  if (ncells == 0) return;
+  // Bytes of partials per cell for each family of the group.
+  auto cell_items = [&](int fid, int k) -> int64_t {
+    const DevFamily& F = L.fam[fid];
+    if (uniform_kind(F.kind)) return F.kind == 0 ? span - 1 : 1;
+    const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+    return B.h_prefix[po + k + 1] - B.h_prefix[po + k];
+  };
+  // Each family's partial region is num_slots x pitch floats, pitch = items rounded up to 256.
+  const int64_t budget_floats = (int64_t)(partial_.bytes / 4);
+  int64_t fixed = 0;
+  for (int fid : R.families) fixed += 256 * (int64_t)L.fam[fid].num_slots;
+  int32_t k0 = 0;
+  while (k0 < ncells) {
+    // Grow the slice while the partial buffer holds it.
+    int32_t k1 = k0;
+    int64_t used = 0;
+    while (k1 < ncells) {
+      int64_t add = 0;
+      for (int fid : R.families) add += cell_items(fid, k1) * L.fam[fid].num_slots;
+      if (k1 > k0 && used + add + fixed > budget_floats) break;
+      used += add;
+      ++k1;
+    }
+    if (used + fixed > budget_floats) throw std::runtime_error("partial buffer too small for one cell");
+    const int32_t kc = k1 - k0;
+    RedArgs ra{};
+    float* base = (float*)partial_.p;
+    size_t off = 0;
+    for (size_t fi = 0; fi < R.families.size(); ++fi) {
+      const int fid = R.families[fi];
+      const DevFamily& F = L.fam[fid];
+      int64_t nitems;
+      if (uniform_kind(F.kind)) {
+        ra.prefix[fi] = nullptr;
+        ra.uniform_count[fi] = F.kind == 0 ? span - 1 : 1;
+        ra.base[fi] = 0;
+        nitems = (int64_t)kc * ra.uniform_count[fi];
+      } else {
+        const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+        ra.prefix[fi] = B.d_prefix + po + k0;
+        ra.base[fi] = B.h_prefix[po + k0];
+        ra.uniform_count[fi] = 1;
+        nitems = B.h_prefix[po + k0 + kc] - ra.base[fi];
+      }
+      const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
+      ra.partial[fi] = base + off;
+      ra.pitch[fi] = pitch;
+      launch_family(B, li, fid, span, use_masks, k0, kc, base + off);
+      off += (size_t)pitch * F.num_slots;
+    }
+    float* inside = (float*)inside_.p;
+    float* outside = (float*)outside_.p;
+    const DevFamily& F0 = L.fam[R.families[0]];
+    ra.chT = F0.t_half >= 2 ? outside : inside;
+    ra.t_half = F0.t_half;
+    ra.cell_offsets = B.d_cell_offsets;
+    ra.lengths = B.d_lengths;
+    ra.cell_list = B.d_cells + B.span_start[span] + k0;
+    ra.off = R.d_off;
+    ra.fam = R.d_fam;
+    ra.slot = R.d_slot;
+    ra.num_cells = kc;
+    ra.cell_size = g.cell_size;
+    ra.n_t = R.n_t;
+    ra.span = span;
+    ra.root_init = (rid == kROutBin);
+    ra.root = g.root;
+    ra.one = g.semiring == kViterbi ? 0.0f : 1.0f;
+    dim3 grid((kc + 63) / 64, (R.n_t + 3) / 4);
+    launch_semi(g.semiring, [&](auto semi) {
+      constexpr int S = decltype(semi)::value;
+      hipLaunchKernelGGL((k_reduce<S>), grid, dim3(64, 4), 0, stream_, ra);
+    });
+    HIP_CHECK(hipGetLastError());
+    k0 = k1;
+  }
+}
+
+void Parser::inside_pass(Batch& B, int32_t li, bool use_masks) {
+  Layer& L = *layers_[li];
+  const Grammar& g = L.g;
+  float* inside = (float*)inside_.p;
+  const float zero = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  const size_t nfl = (size_t)B.num_cells * g.cell_size;
+  hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, inside, nfl, zero);
+  if (g.semiring == kReal && !B.has_scales) {
+    hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream_, (float*)in_scale_.p, (size_t)B.num_cells, 0.0f);
+    hipLaunchKernelGGL(k_fill, dim3(256), dim3(256), 0, stream_, (float*)out_scale_.p, (size_t)B.num_cells, 0.0f);
+  }
+  launch_semi(g.semiring, [&](auto semi) {
+    constexpr int S = decltype(semi)::value;
+    hipLaunchKernelGGL((k_tags<S>), dim3((B.total_len + 127) / 128), dim3(128), 0, stream_, inside,
+                       B.d_cell_offsets, B.d_lengths, B.d_len_offsets, B.d_words, B.n, B.total_len,
+                       L.d_lex_offsets, L.d_lex_tags, L.d_lex_logp, (const float*)in_scale_.p,
+                       (double*)pos_scale_.p, g.cell_size);
+  });
+  HIP_CHECK(hipGetLastError());
+  run_group(B, li, kRInUT, 1, use_masks);
+  for (int w = 2; w <= B.max_len; ++w) {
+    run_group(B, li, w == 2 ? kRInBin2 : kRInBin, w, use_masks);
+    run_group(B, li, kRInUN, w, use_masks);
+  }
+}
+
+void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
+  Layer& L = *layers_[li];
+  const Grammar& g = L.g;
+  const float zero = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  const size_t nfl = (size_t)B.num_cells * g.cell_size;
+  hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, (float*)outside_.p, nfl, zero);
+  for (int w = B.max_len; w >= 1; --w) {
+    run_group(B, li, kROutBin, w, use_masks);
+    run_group(B, li, w >= 2 ? kROutUN : kROutW1, w, use_masks);
+  }
+}
+
+// what: 0 = parse, 1 = partitions, 2 = insideOutside, 3 = masks of the last coarse layer,
+// 4 = debug charts.
+void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t first, int32_t count,
+                       int32_t what, std::vector<std::vector<TreeNode>>* viterbi,
+                       std::vector<std::vector<MbrNode>>* mbr, float* partitions, Charts* charts,
+                       int32_t chart_layer, std::vector<uint32_t>* masks_out) {
+  Batch B;
+  B.n = count;
+  B.cell_offsets.push_back(0);
+  B.len_offsets.push_back(0);
+  B.tree_offsets.push_back(0);
+  for (int32_t i = 0; i < count; ++i) {
+    const int32_t len = offsets[first + i + 1] - offsets[first + i];
+    if (len < 1 || len > kMaxLength) throw std::runtime_error("sentence length must be in [1, 255]");
+    B.lengths.push_back(len);
+    B.max_len = std::max(B.max_len, len);
+    B.cell_offsets.push_back(B.cell_offsets.back() + len * (len + 1));
+    B.len_offsets.push_back(B.len_offsets.back() + len);
+    B.tree_offsets.push_back(B.tree_offsets.back() + 2 * len - 1);
+    for (int32_t j = 0; j < len; ++j) {
+      const int32_t w = words[offsets[first + i] + j];
+      for (auto& l : layers_)
+        if (w < 0 || w + 1 >= (int32_t)l->g.lex_offsets.size())
+          throw std::runtime_error("word id out of range of the grammar lexicon");
+      B.words.push_back(w);
+    }
+  }
+  B.num_cells = B.cell_offsets.back();
+  B.total_len = B.len_offsets.back();
+  // Cell lists per span: sentence-major, begin-minor.
+  B.span_start.assign(B.max_len + 2, 0);
+  B.span_count.assign(B.max_len + 2, 0);
+  for (int w = 1; w <= B.max_len; ++w) {
+    B.span_start[w] = (int32_t)B.h_cells.size();
+    for (int32_t i = 0; i < count; ++i)
+      for (int b = 0; b + w <= B.lengths[i]; ++b) B.h_cells.push_back((i << 8) | b);
+    B.span_count[w] = (int32_t)B.h_cells.size() - B.span_start[w];
+  }
+  const bool need_outside = (what != 1) && !(what == 0 && layers_.back()->g.semiring == kViterbi &&
+                                             layers_.size() == 1);
+  B.prefix_off.assign((size_t)(B.max_len + 2) * kNumFamilies, -1);
+  if (what != 1 || layers_.size() > 1) {
+    for (int w = 1; w <= B.max_len; ++w)
+      for (int fid = kFOutLNN; fid <= kFOutRTT; ++fid) {
+        const DevFamily& F = layers_[0]->fam[fid];
+        B.prefix_off[(size_t)w * kNumFamilies + fid] = (int64_t)B.h_prefix.size();
+        int32_t acc = 0;
+        B.h_prefix.push_back(0);
+        for (int k = 0; k < B.span_count[w]; ++k) {
+          const int32_t packed = B.h_cells[B.span_start[w] + k];
+          const int i = packed >> 8, b = packed & 255;
+          acc += items_for(F.kind, b, b + w, B.lengths[i]);
+          B.h_prefix.push_back(acc);
+        }
+      }
+  }
+  (void)need_outside;
+
+  // Device buffers for the batch plan.
+  std::vector<int32_t> plan;
+  auto put = [&](const std::vector<int32_t>& v) {
+    size_t o = plan.size();
+    plan.insert(plan.end(), v.begin(), v.end());
+    return o;
+  };
+  const size_t o_len = put(B.lengths), o_co = put(B.cell_offsets), o_lo = put(B.len_offsets),
+               o_w = put(B.words), o_to = put(B.tree_offsets), o_c = put(B.h_cells),
+               o_p = put(B.h_prefix);
+  ensure(plan_, plan.size() * 4 + 256);
+  HIP_CHECK(hipMemcpyAsync(plan_.p, plan.data(), plan.size() * 4, hipMemcpyHostToDevice, stream_));
+  int32_t* dp = (int32_t*)plan_.p;
+  B.d_lengths = dp + o_len;
+  B.d_cell_offsets = dp + o_co;
+  B.d_len_offsets = dp + o_lo;
+  B.d_words = dp + o_w;
+  B.d_tree_offsets = dp + o_to;
+  B.d_cells = dp + o_c;
+  B.d_prefix = dp + o_p;
+
+  int32_t max_cell = 0, mask_words = layers_[0]->g.mask_words;
+  for (auto& l : layers_) max_cell = std::max(max_cell, l->g.cell_size);
+  ensure(inside_, (size_t)B.num_cells * max_cell * 4);
+  ensure(outside_, (size_t)B.num_cells * max_cell * 4);
+  ensure(in_scale_, (size_t)B.num_cells * 4);
+  ensure(out_scale_, (size_t)B.num_cells * 4);
+  ensure(masks_, (size_t)2 * B.num_cells * mask_words * 4);
+  ensure(pos_scale_, (size_t)B.total_len * 8);
+  const size_t partial_bytes = std::max<size_t>((size_t)(mem_bytes_ * 0.3), (size_t)64 << 20);
+  ensure(partial_, partial_bytes);
+
+  const int32_t nl = (int32_t)layers_.size();
+  const int32_t last_layer = (what == 3) ? nl - 2 : nl - 1;
+  B.has_masks = false;
+  B.has_scales = false;
+  for (int32_t li = 0; li <= last_layer; ++li) {
+    Layer& L = *layers_[li];
+    const Grammar& g = L.g;
+    const bool is_last = (li == nl - 1);
+    const bool use_masks = B.has_masks;
+    timed_begin();
+    inside_pass(B, li, use_masks);
+    double dummy = 0;
+    timed_end(&dummy);
+    stats.total_ms += dummy;
+    if (!is_last) {
+      timed_begin();
+      outside_pass(B, li, use_masks);
+      // computePruningMasks: masks at threshold -9 (CLParser.scala:223-229).
+      const int32_t mask_out = B.has_masks ? 1 - B.mask_in : 0;
+      uint32_t* mo = (uint32_t*)masks_.p + (size_t)mask_out * B.num_cells * mask_words;
+      hipLaunchKernelGGL(k_masks, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_, mo,
+                         (const float*)inside_.p, (const float*)outside_.p, B.d_cell_offsets,
+                         B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n, g.num_syms,
+                         g.cell_size, g.root, -9.0f, mask_words);
+      HIP_CHECK(hipGetLastError());
+      if (layers_[li + 1]->g.semiring == kReal) {
+        hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+                           (float*)in_scale_.p, (const float*)inside_.p, g.cell_size, B.num_cells);
+        hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+                           (float*)out_scale_.p, (const float*)outside_.p, g.cell_size, B.num_cells);
+        B.has_scales = true;
+      } else {
+        B.has_scales = false;
+      }
+      B.mask_in = mask_out;
+      B.has_masks = true;
+      timed_end(&dummy);
+      stats.total_ms += dummy;
+      if (what == 4 && li == chart_layer) break;
+      continue;
+    }
+    // Final layer.
+    if (what == 1) {
+      std::vector<float> roots(B.n);
+      ensure(aux_, (size_t)B.n * 4);
+      hipLaunchKernelGGL(k_get_root, dim3((B.n + 127) / 128), dim3(128), 0, stream_, (float*)aux_.p,
+                         (const float*)inside_.p, B.d_cell_offsets, B.n, g.cell_size, g.root);
+      HIP_CHECK(hipMemcpyAsync(roots.data(), aux_.p, (size_t)B.n * 4, hipMemcpyDeviceToHost, stream_));
+      std::vector<double> ps(B.total_len, 0.0);
+      std::vector<float> isc(B.num_cells, 0.0f);
+      if (g.semiring == kReal) {
+        HIP_CHECK(hipMemcpyAsync(ps.data(), pos_scale_.p, (size_t)B.total_len * 8, hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipMemcpyAsync(isc.data(), in_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost, stream_));
+      }
+      HIP_CHECK(hipStreamSynchronize(stream_));
+      for (int32_t i = 0; i < B.n; ++i) {
+        if (g.semiring == kReal) {
+          double sc = 0;
+          for (int j = B.len_offsets[i]; j < B.len_offsets[i + 1]; ++j) sc += ps[j];
+          const int n = B.lengths[i];
+          const float top_scale = isc[B.cell_offsets[i] + tri(n) + chart_index(0, n, n)];
+          // RealSemiring.toLogSpace(x, scale) + partitionScales (CLParser.scala:197-199)
+          const float ls = (float)std::log((double)roots[i]) + top_scale;
+          partitions[first + i] = ls + (float)sc;
+        } else {
+          partitions[first + i] = roots[i];
+        }
+      }
+      break;
+    }
+    if (g.semiring == kViterbi) {
+      if (what == 2 || what == 4) {
+        timed_begin();
+        outside_pass(B, li, use_masks);
+        timed_end(&dummy);
+        stats.total_ms += dummy;
+      }
+      if (what == 0 || what == 2) {
+        ensure(trees_, (size_t)B.tree_offsets.back() * sizeof(TreeNode));
+        timed_begin();
+        hipLaunchKernelGGL(k_viterbi, dim3(B.n), dim3(64), 0, stream_, (TreeNode*)trees_.p,
+                           B.d_tree_offsets, (const float*)inside_.p, B.d_cell_offsets, B.d_lengths,
+                           B.n, g.cell_size, g.root, g.num_nonterms(), L.d_dec_offsets,
+                           L.d_dec_scores, L.d_dec_left, L.d_dec_right);
+        HIP_CHECK(hipGetLastError());
+        timed_end(&stats.other_ms);
+        if (what == 0) {
+          std::vector<TreeNode> all(B.tree_offsets.back());
+          HIP_CHECK(hipMemcpyAsync(all.data(), trees_.p, all.size() * sizeof(TreeNode), hipMemcpyDeviceToHost, stream_));
+          HIP_CHECK(hipStreamSynchronize(stream_));
+          for (int32_t i = 0; i < B.n; ++i)
+            (*viterbi)[first + i].assign(all.begin() + B.tree_offsets[i], all.begin() + B.tree_offsets[i + 1]);
+        }
+      }
+    } else {
+      timed_begin();
+      outside_pass(B, li, use_masks);
+      timed_end(&dummy);
+      stats.total_ms += dummy;
+      if (what == 0 || what == 2) {
+        ensure(mbr_, (size_t)B.num_cells * sizeof(MbrCell));
+        hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+                           (MbrCell*)mbr_.p, (const float*)inside_.p, (const float*)outside_.p,
+                           B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n,
+                           g.num_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size());
+        HIP_CHECK(hipGetLastError());
+      }
+      if (what == 0) {
+        std::vector<MbrCell> dec(B.num_cells);
+        std::vector<float> isc(B.num_cells), osc(B.num_cells);
+        HIP_CHECK(hipMemcpyAsync(dec.data(), mbr_.p, dec.size() * sizeof(MbrCell), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipMemcpyAsync(isc.data(), in_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipMemcpyAsync(osc.data(), out_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        // extractMBRParses (CLParser.scala:589-696).
+        for (int32_t i = 0; i < B.n; ++i) {
+          const int n = B.lengths[i];
+          const int co = B.cell_offsets[i];
+          auto bot_cell = [&](int b, int e) { return co + chart_index(b, e, n); };
+          auto top_cell = [&](int b, int e) { return co + tri(n) + chart_index(b, e, n); };
+          const int W = n + 1;
+          std::vector<float> best(W * W, 0.0f);
+          std::vector<int> split(W * W, 0);
+          std::vector<char> use_unary(W * W, 0);
+          const float root_top_scale = isc[top_cell(0, n)];
+          for (int span = 1; span <= n; ++span)
+            for (int b = 0; b + span <= n; ++b) {
+              const int e = b + span;
+              float top = dec[top_cell(b, e)].score, bot = dec[bot_cell(b, e)].score;
+              if (B.has_scales) {
+                top *= (float)std::exp((double)(isc[top_cell(b, e)] + osc[top_cell(b, e)] - root_top_scale));
+                bot *= (float)std::exp((double)(isc[bot_cell(b, e)] + osc[bot_cell(b, e)] - root_top_scale));
+              }
+              if (std::isnan(top)) top = 0.0f;
+              if (std::isnan(bot)) bot = 0.0f;
+              top = std::min(top, 1.0f);
+              bot = std::min(bot, 1.0f);
+              const bool uu = (b + 1 < e) || top > 0.4;
+              use_unary[b * W + e] = uu;
+              float sc = uu ? top + bot : bot;
+              if (span > 1) {
+                float bs = 0.0f;
+                int bsplit = b + 1;
+                for (int s = b + 1; s < e; ++s) {
+                  const float v = best[b * W + s] + best[s * W + e];
+                  if (v > bs) {
+                    bsplit = s;
+                    bs = v;
+                  }
+                }
+                split[b * W + e] = bsplit;
+                sc += bs;
+              }
+              best[b * W + e] = sc;
+            }
+          std::vector<MbrNode>& out = (*mbr)[first + i];
+          out.clear();
+          std::vector<std::pair<int, int>> stack{{0, n}};
+          while (!stack.empty()) {
+            auto [b, e] = stack.back();
+            stack.pop_back();
+            MbrNode nd;
+            nd.begin = b;
+            nd.end = e;
+            nd.bot_coarse = dec[bot_cell(b, e)].symbol;
+            nd.top_coarse = use_unary[b * (n + 1) + e] ? dec[top_cell(b, e)].symbol : -1;
+            nd.score = best[b * (n + 1) + e];
+            out.push_back(nd);
+            if (e - b > 1) {
+              const int s = split[b * (n + 1) + e];
+              stack.push_back({s, e});
+              stack.push_back({b, s});
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if (charts != nullptr) {
+    const Grammar& g = layers_[chart_layer]->g;
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    charts->cell_size = g.cell_size;
+    charts->inside.resize((size_t)B.num_cells * g.cell_size);
+    charts->outside.resize((size_t)B.num_cells * g.cell_size);
+    charts->in_scale.resize(B.num_cells);
+    charts->out_scale.resize(B.num_cells);
+    charts->cell_offsets = B.cell_offsets;
+    HIP_CHECK(hipMemcpy(charts->inside.data(), inside_.p, charts->inside.size() * 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(charts->outside.data(), outside_.p, charts->outside.size() * 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(charts->in_scale.data(), in_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(charts->out_scale.data(), out_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost));
+    if (B.has_masks) {
+      charts->masks.resize((size_t)B.num_cells * mask_words);
+      HIP_CHECK(hipMemcpy(charts->masks.data(), (uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * mask_words,
+                          charts->masks.size() * 4, hipMemcpyDeviceToHost));
+    }
+    if (g.semiring == kReal) {
+      std::vector<double> ps(B.total_len);
+      HIP_CHECK(hipMemcpy(ps.data(), pos_scale_.p, (size_t)B.total_len * 8, hipMemcpyDeviceToHost));
+      charts->partition_scales.assign(B.n, 0.0);
+      for (int32_t i = 0; i < B.n; ++i)
+        for (int j = B.len_offsets[i]; j < B.len_offsets[i + 1]; ++j) charts->partition_scales[i] += ps[j];
+    }
+  }
+  if (masks_out != nullptr && B.has_masks) {
+    std::vector<uint32_t> m((size_t)B.num_cells * mask_words);
+    HIP_CHECK(hipMemcpyAsync(m.data(), (uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * mask_words,
+                             m.size() * 4, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    masks_out->insert(masks_out->end(), m.begin(), m.end());
+  }
+  flush_timing();
+}
+
+void Parser::parse(const int32_t* words, const int32_t* offsets, int32_t n,
+                   std::vector<std::vector<TreeNode>>* viterbi, std::vector<std::vector<MbrNode>>* mbr) {
+  HIP_CHECK(hipSetDevice(device_));
+  stats = PassStats{};
+  if (viterbi) viterbi->assign(n, {});
+  if (mbr) mbr->assign(n, {});
+  if (layers_.back()->g.semiring == kViterbi && !viterbi) throw std::runtime_error("viterbi output required");
+  if (layers_.back()->g.semiring == kReal && !mbr) throw std::runtime_error("mbr output required");
+  for (auto [f, c] : make_batches(offsets, n)) run_batch(words, offsets, f, c, 0, viterbi, mbr, nullptr, nullptr, 0, nullptr);
+}
+
+void Parser::partitions(const int32_t* words, const int32_t* offsets, int32_t n, float* out) {
+  HIP_CHECK(hipSetDevice(device_));
+  stats = PassStats{};
+  for (auto [f, c] : make_batches(offsets, n)) run_batch(words, offsets, f, c, 1, nullptr, nullptr, out, nullptr, 0, nullptr);
+}
+
+void Parser::inside_outside(const int32_t* words, const int32_t* offsets, int32_t n) {
+  HIP_CHECK(hipSetDevice(device_));
+  stats = PassStats{};
+  for (auto [f, c] : make_batches(offsets, n)) run_batch(words, offsets, f, c, 2, nullptr, nullptr, nullptr, nullptr, 0, nullptr);
+}
+
+void Parser::masks(const int32_t* words, const int32_t* offsets, int32_t n, std::vector<uint32_t>* out,
+                   std::vector<int32_t>* cell_offsets) {
+  HIP_CHECK(hipSetDevice(device_));
+  if (layers_.size() < 2) throw std::runtime_error("masks need a coarse layer");
+  stats = PassStats{};
+  out->clear();
+  cell_offsets->assign(1, 0);
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t len = offsets[i + 1] - offsets[i];
+    cell_offsets->push_back(cell_offsets->back() + len * (len + 1));
+  }
+  for (auto [f, c] : make_batches(offsets, n)) run_batch(words, offsets, f, c, 3, nullptr, nullptr, nullptr, nullptr, 0, out);
+}
+
+void Parser::debug_charts(const int32_t* words, const int32_t* offsets, int32_t n, int32_t layer,
+                          bool outside, Charts* out) {
+  HIP_CHECK(hipSetDevice(device_));
+  stats = PassStats{};
+  if (layer < 0 || layer >= (int32_t)layers_.size()) throw std::runtime_error("bad layer index");
+  // Charts of one batch only: the caller keeps the sentences within one batch.
+  auto batches = make_batches(offsets, n);
+  if (batches.size() != 1) throw std::runtime_error("debug_charts input does not fit one batch");
+  // Run layers up to `layer`; for the final layer run inside (and outside if asked).
+  const int32_t what = (layer == (int32_t)layers_.size() - 1) ? (outside ? 4 : 1) : 4;
+  std::vector<float> parts(n);
+  run_batch(words, offsets, 0, n, what, nullptr, nullptr, parts.data(), out, layer, nullptr);
+}
+
+}  // namespace puck
