@@ -50,7 +50,8 @@ class GrammarInfo(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("rule_kernel_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double),
                 ("other_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("rule_apps", ctypes.c_double), ("rule_launches", ctypes.c_int64)]
+                ("rule_apps", ctypes.c_double), ("rule_launches", ctypes.c_int64),
+                ("device_ms", ctypes.c_double)]
 
 
 # Exported symbols and their signatures: (name, restype, argtypes).
@@ -84,6 +85,7 @@ SIGNATURES = [
     ("puck_debug_charts", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            _F32P, _F32P, _F32P, _F32P, _U32P, _F64P, ctypes.c_int64]),
     ("puck_parser_stats", ctypes.c_int32, [_P, ctypes.POINTER(Stats)]),
+    ("puck_parser_launches", ctypes.c_int64, [_P, _F32P, _F64P, _I32P, ctypes.c_int64]),
 ]
 
 
@@ -290,3 +292,13 @@ class CLParser:
         s = Stats()
         _check(lib().puck_parser_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def launches(self):
+        """Per rule-kernel launch of the last call: (ms float32[], rule_apps float64[], family int32[])."""
+        n = lib().puck_parser_launches(self._h, None, None, None, 0)
+        ms = np.zeros(max(n, 1), np.float32)
+        apps = np.zeros(max(n, 1), np.float64)
+        fam = np.zeros(max(n, 1), np.int32)
+        lib().puck_parser_launches(self._h, ms.ctypes.data_as(_F32P), apps.ctypes.data_as(_F64P),
+                                   fam.ctypes.data_as(_I32P), n)
+        return ms[:n], apps[:n], fam[:n]

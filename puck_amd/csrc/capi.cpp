@@ -319,7 +319,20 @@ int32_t puck_parser_stats(const puck_parser_t* h, puck_stats_t* out) {
   out->total_ms = s.total_ms;
   out->rule_apps = s.rule_apps;
   out->rule_launches = s.rule_launches;
+  out->device_ms = s.device_ms;
   return PUCK_OK;
+}
+
+int64_t puck_parser_launches(const puck_parser_t* h, float* ms, double* apps, int32_t* family, int64_t cap) {
+  if (!h) return set_error(PUCK_ERR_ARG, "bad argument");
+  const puck::PassStats& s = h->p->stats;
+  const int64_t n = (int64_t)s.launch_ms.size();
+  for (int64_t i = 0; i < n && i < cap; ++i) {
+    if (ms) ms[i] = s.launch_ms[i];
+    if (apps) apps[i] = s.launch_apps[i];
+    if (family) family[i] = s.launch_family[i];
+  }
+  return n;
 }
 
 }  // extern "C"

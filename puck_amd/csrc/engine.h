@@ -123,7 +123,8 @@ struct PassStats {
   double other_ms = 0;
   double rule_apps = 0;        // algorithmic rule applications executed by rule kernels
   int64_t rule_launches = 0;
-  double total_ms = 0;
+  double total_ms = 0;          // device time of the passes (inside, outside, decode)
+  double device_ms = 0;         // device time from input upload to the first result download
   std::vector<float> launch_ms;      // per rule-kernel launch duration
   std::vector<double> launch_apps;   // per rule-kernel launch rule applications
   std::vector<int32_t> launch_family;
@@ -177,6 +178,12 @@ class Parser {
   std::vector<hipEvent_t> ev_pool_;
   size_t ev_next_ = 0;
   std::vector<TimedLaunch> pending_;
+  struct Phase { hipEvent_t a, b; double* acc; };
+  std::vector<Phase> phases_;
+  hipEvent_t phase_start_ = nullptr;
+  hipEvent_t dev_start_ = nullptr;
+  bool dev_marked_ = true;
+  void mark_device_end();
   hipEvent_t next_event();
   void flush_timing();
 

@@ -908,13 +908,15 @@ std::vector<std::pair<int32_t, int32_t>> Parser::make_batches(const int32_t* off
   return out;
 }
 
-void Parser::timed_begin() { HIP_CHECK(hipEventRecord(ev0_, stream_)); }
+// Phase timing with pooled events, resolved once per batch in flush_timing (no host syncs).
+void Parser::timed_begin() {
+  phase_start_ = next_event();
+  HIP_CHECK(hipEventRecord(phase_start_, stream_));
+}
 void Parser::timed_end(double* acc) {
-  HIP_CHECK(hipEventRecord(ev1_, stream_));
-  HIP_CHECK(hipEventSynchronize(ev1_));
-  float ms = 0;
-  HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-  *acc += ms;
+  hipEvent_t e = next_event();
+  HIP_CHECK(hipEventRecord(e, stream_));
+  phases_.push_back(Phase{phase_start_, e, acc});
 }
 
 namespace {
@@ -1021,6 +1023,14 @@ hipEvent_t Parser::next_event() {
   return ev_pool_[ev_next_++];
 }
 
+void Parser::mark_device_end() {
+  if (dev_marked_) return;
+  dev_marked_ = true;
+  hipEvent_t e = next_event();
+  HIP_CHECK(hipEventRecord(e, stream_));
+  phases_.push_back(Phase{dev_start_, e, &stats.device_ms});
+}
+
 void Parser::flush_timing() {
   HIP_CHECK(hipStreamSynchronize(stream_));
   for (const TimedLaunch& t : pending_) {
@@ -1031,7 +1041,13 @@ void Parser::flush_timing() {
     stats.launch_apps.push_back(t.apps);
     stats.launch_family.push_back(t.family);
   }
+  for (const Phase& ph : phases_) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, ph.a, ph.b));
+    *ph.acc += ms;
+  }
   pending_.clear();
+  phases_.clear();
   ev_next_ = 0;
 }
 
@@ -1227,6 +1243,9 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
                o_p = put(B.h_prefix);
   ensure(plan_, plan.size() * 4 + 256);
   HIP_CHECK(hipMemcpyAsync(plan_.p, plan.data(), plan.size() * 4, hipMemcpyHostToDevice, stream_));
+  dev_start_ = next_event();
+  HIP_CHECK(hipEventRecord(dev_start_, stream_));
+  dev_marked_ = false;
   int32_t* dp = (int32_t*)plan_.p;
   B.d_lengths = dp + o_len;
   B.d_cell_offsets = dp + o_co;
@@ -1258,9 +1277,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
     const bool use_masks = B.has_masks;
     timed_begin();
     inside_pass(B, li, use_masks);
-    double dummy = 0;
-    timed_end(&dummy);
-    stats.total_ms += dummy;
+    timed_end(&stats.total_ms);
     if (!is_last) {
       timed_begin();
       outside_pass(B, li, use_masks);
@@ -1283,8 +1300,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
       }
       B.mask_in = mask_out;
       B.has_masks = true;
-      timed_end(&dummy);
-      stats.total_ms += dummy;
+      timed_end(&stats.total_ms);
       if (what == 4 && li == chart_layer) break;
       continue;
     }
@@ -1294,6 +1310,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
       ensure(aux_, (size_t)B.n * 4);
       hipLaunchKernelGGL(k_get_root, dim3((B.n + 127) / 128), dim3(128), 0, stream_, (float*)aux_.p,
                          (const float*)inside_.p, B.d_cell_offsets, B.n, g.cell_size, g.root);
+      mark_device_end();
       HIP_CHECK(hipMemcpyAsync(roots.data(), aux_.p, (size_t)B.n * 4, hipMemcpyDeviceToHost, stream_));
       std::vector<double> ps(B.total_len, 0.0);
       std::vector<float> isc(B.num_cells, 0.0f);
@@ -1321,8 +1338,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
       if (what == 2 || what == 4) {
         timed_begin();
         outside_pass(B, li, use_masks);
-        timed_end(&dummy);
-        stats.total_ms += dummy;
+        timed_end(&stats.total_ms);
       }
       if (what == 0 || what == 2) {
         ensure(trees_, (size_t)B.tree_offsets.back() * sizeof(TreeNode));
@@ -1335,6 +1351,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
         timed_end(&stats.other_ms);
         if (what == 0) {
           std::vector<TreeNode> all(B.tree_offsets.back());
+          mark_device_end();
           HIP_CHECK(hipMemcpyAsync(all.data(), trees_.p, all.size() * sizeof(TreeNode), hipMemcpyDeviceToHost, stream_));
           HIP_CHECK(hipStreamSynchronize(stream_));
           for (int32_t i = 0; i < B.n; ++i)
@@ -1344,8 +1361,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
     } else {
       timed_begin();
       outside_pass(B, li, use_masks);
-      timed_end(&dummy);
-      stats.total_ms += dummy;
+      timed_end(&stats.total_ms);
       if (what == 0 || what == 2) {
         ensure(mbr_, (size_t)B.num_cells * sizeof(MbrCell));
         hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
@@ -1357,6 +1373,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
       if (what == 0) {
         std::vector<MbrCell> dec(B.num_cells);
         std::vector<float> isc(B.num_cells), osc(B.num_cells);
+        mark_device_end();
         HIP_CHECK(hipMemcpyAsync(dec.data(), mbr_.p, dec.size() * sizeof(MbrCell), hipMemcpyDeviceToHost, stream_));
         HIP_CHECK(hipMemcpyAsync(isc.data(), in_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost, stream_));
         HIP_CHECK(hipMemcpyAsync(osc.data(), out_scale_.p, (size_t)B.num_cells * 4, hipMemcpyDeviceToHost, stream_));
@@ -1426,6 +1443,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
     }
   }
 
+  mark_device_end();
   if (charts != nullptr) {
     const Grammar& g = layers_[chart_layer]->g;
     HIP_CHECK(hipStreamSynchronize(stream_));
