@@ -134,6 +134,27 @@ int32_t puck_parser_stats(const puck_parser_t* p, puck_stats_t* out);
  * (0-3 inside NN/NT/TN/TT, 4-7 outside-left, 8-11 outside-right, 12-15 unary). Returns the count. */
 int64_t puck_parser_launches(const puck_parser_t* p, float* ms, double* apps, int32_t* family, int64_t cap);
 
+/* ---- serving (replaces BatchFunctionAnnotatorService, AnnotatorService.scala:29-78) ----
+ * Continuous batching: up to max_inflight batches run concurrently (one HIP stream each); a free
+ * slot admits queued requests in arrival order while their chart cells fit its memory budget. */
+typedef struct puck_server puck_server_t;
+typedef struct {
+  int64_t completed, batches;
+  double mean_batch, p50_ms, p90_ms, p99_ms, max_ms;
+} puck_server_stats_t;
+int32_t puck_server_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device,
+                           int64_t mem_bytes, int32_t max_inflight, int32_t max_batch, int32_t flags,
+                           puck_server_t** out);
+void puck_server_free(puck_server_t* s);
+/* Enqueues one sentence (word ids); writes its ticket. Thread-safe. */
+int32_t puck_server_submit(puck_server_t* s, const int32_t* words, int32_t len, int64_t* ticket);
+/* Result of a ticket: returns 1 when done (nodes written: 2n-1 tree nodes, Viterbi final layer, or
+ * MBR nodes for a Real final layer, cast to the matching struct), 0 when still pending (wait = 0),
+ * negative on failure. */
+int32_t puck_server_result(puck_server_t* s, int64_t ticket, int32_t wait, void* nodes, int64_t cap);
+int32_t puck_server_flush(puck_server_t* s);
+int32_t puck_server_stats(puck_server_t* s, int32_t reset, puck_server_stats_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,7 +86,20 @@ SIGNATURES = [
                                            _F32P, _F32P, _F32P, _F32P, _U32P, _F64P, ctypes.c_int64]),
     ("puck_parser_stats", ctypes.c_int32, [_P, ctypes.POINTER(Stats)]),
     ("puck_parser_launches", ctypes.c_int64, [_P, _F32P, _F64P, _I32P, ctypes.c_int64]),
+    ("puck_server_create", ctypes.c_int32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
+    ("puck_server_free", None, [_P]),
+    ("puck_server_submit", ctypes.c_int32, [_P, _I32P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    ("puck_server_result", ctypes.c_int32, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
+    ("puck_server_flush", ctypes.c_int32, [_P]),
+    ("puck_server_stats", ctypes.c_int32, [_P, ctypes.c_int32, ctypes.c_void_p]),
 ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_int64), ("batches", ctypes.c_int64), ("mean_batch", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p90_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double)]
 
 
 def lib():
@@ -302,3 +315,49 @@ class CLParser:
         lib().puck_parser_launches(self._h, ms.ctypes.data_as(_F32P), apps.ctypes.data_as(_F64P),
                                    fam.ctypes.data_as(_I32P), n)
         return ms[:n], apps[:n], fam[:n]
+
+
+class Server:
+    """Continuous-batching parse server (replaces BatchFunctionAnnotatorService,
+    AnnotatorService.scala:29-78): submit sentences from any thread, collect results by ticket."""
+
+    def __init__(self, layers, device=0, mem_bytes=16 << 30, max_inflight=2, max_batch=20000, items_per_lane=2):
+        self.layers = list(layers)
+        arr = (_P * len(self.layers))(*[g._h for g in self.layers])
+        h = _P()
+        flags = 1 if items_per_lane == 1 else 0
+        _check(lib().puck_server_create(arr, len(self.layers), device, mem_bytes, max_inflight, max_batch, flags,
+                                        ctypes.byref(h)))
+        self._h = h
+        self._mbr = self.layers[-1].semiring == REAL
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.puck_server_free(self._h)
+            self._h = None
+
+    def submit(self, words):
+        w = np.ascontiguousarray(words, np.int32)
+        t = ctypes.c_int64()
+        _check(lib().puck_server_submit(self._h, w.ctypes.data_as(_I32P), len(w), ctypes.byref(t)))
+        return t.value
+
+    def result(self, ticket, n_words, wait=True):
+        """Tree of a finished request (2n-1 nodes), or None if still pending and wait is False."""
+        k = 2 * n_words - 1
+        if self._mbr:
+            buf = (MbrNode * k)()
+        else:
+            buf = (TreeNode * k)()
+        rc = _check(lib().puck_server_result(self._h, ticket, int(wait), ctypes.cast(buf, ctypes.c_void_p), k))
+        if rc == 0:
+            return None
+        return np.ctypeslib.as_array(buf).view(np.int32).reshape(k, -1).copy()
+
+    def flush(self):
+        _check(lib().puck_server_flush(self._h))
+
+    def stats(self, reset=False):
+        s = ServerStats()
+        _check(lib().puck_server_stats(self._h, int(reset), ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}

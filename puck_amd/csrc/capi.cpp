@@ -8,6 +8,7 @@
 
 #include "../../include/puck.h"
 #include "engine.h"
+#include "server.h"
 #include "synth.h"
 
 struct puck_grammar {
@@ -16,6 +17,11 @@ struct puck_grammar {
 
 struct puck_parser {
   std::unique_ptr<puck::Parser> p;
+};
+
+struct puck_server {
+  std::unique_ptr<puck::Server> s;
+  bool mbr = false;
 };
 
 namespace {
@@ -333,6 +339,88 @@ int64_t puck_parser_launches(const puck_parser_t* h, float* ms, double* apps, in
     if (family) family[i] = s.launch_family[i];
   }
   return n;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int32_t puck_server_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device,
+                           int64_t mem_bytes, int32_t max_inflight, int32_t max_batch, int32_t flags,
+                           puck_server_t** out) {
+  return guarded([&]() -> int32_t {
+    if (!layers || n_layers < 1 || !out || max_inflight < 1) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<std::shared_ptr<puck::Grammar>> gs;
+    for (int32_t i = 0; i < n_layers; ++i) {
+      if (!layers[i]) return set_error(PUCK_ERR_ARG, "null grammar layer");
+      gs.push_back(layers[i]->g);
+    }
+    auto* h = new puck_server;
+    h->mbr = gs.back()->semiring == puck::kReal;
+    try {
+      h->s.reset(new puck::Server(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, max_inflight,
+                                  max_batch > 0 ? max_batch : 20000, (flags & 1) ? 1 : 2));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return PUCK_OK;
+  });
+}
+
+void puck_server_free(puck_server_t* s) { delete s; }
+
+int32_t puck_server_submit(puck_server_t* h, const int32_t* words, int32_t len, int64_t* ticket) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !ticket) return set_error(PUCK_ERR_ARG, "bad argument");
+    *ticket = h->s->submit(words, len);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_server_result(puck_server_t* h, int64_t ticket, int32_t wait, void* nodes, int64_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<puck::TreeNode> t;
+    std::vector<puck::MbrNode> m;
+    int32_t status = 0;
+    std::string err;
+    if (!h->s->result(ticket, wait != 0, &t, &m, &status, &err)) return 0;
+    if (status < 0) return set_error(status, err);
+    if (h->mbr) {
+      if ((int64_t)m.size() > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
+      if (nodes) std::memcpy(nodes, m.data(), m.size() * sizeof(puck_mbr_node_t));
+    } else {
+      if ((int64_t)t.size() > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
+      if (nodes) std::memcpy(nodes, t.data(), t.size() * sizeof(puck_tree_node_t));
+    }
+    return 1;
+  });
+}
+
+int32_t puck_server_flush(puck_server_t* h) {
+  return guarded([&]() -> int32_t {
+    if (!h) return set_error(PUCK_ERR_ARG, "bad argument");
+    h->s->flush();
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_server_stats(puck_server_t* h, int32_t reset, puck_server_stats_t* out) {
+  return guarded([&]() -> int32_t {
+    if (!h || !out) return set_error(PUCK_ERR_ARG, "bad argument");
+    puck::ServerStats s = h->s->stats();
+    out->completed = s.completed;
+    out->batches = s.batches;
+    out->mean_batch = s.mean_batch;
+    out->p50_ms = s.p50_ms;
+    out->p90_ms = s.p90_ms;
+    out->p99_ms = s.p99_ms;
+    out->max_ms = s.max_ms;
+    if (reset) h->s->reset_stats();
+    return PUCK_OK;
+  });
 }
 
 }  // extern "C"

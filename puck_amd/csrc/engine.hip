@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   constexpr int ITEMS = 64 * IPL;
   extern __shared__ __align__(16) float smem[];
   __shared__ int32_t s_cell_a[ITEMS], s_cell_b[ITEMS];
-  __shared__ uint32_t s_gate[ITEMS];
+  __shared__ uint32_t s_gate[4][ITEMS];  // 128 block bits per item
   __shared__ float s_scale[ITEMS];
   __shared__ int32_t s_next;
   const float zero = semi_zero<SEMI>();
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   if (tid < ITEMS) {
     const int item = tile0 + tid;
     int ca = 0, cb = 0;
-    uint32_t gate = 0;
+    uint32_t gate[4] = {0u, 0u, 0u, 0u};
     float scale = 1.0f;
     if (item < a.num_items) {
       int k, j;
@@ -128,11 +128,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
       cb = UNARY ? 0 : cell_abs(a.cell_offsets, sent, n, a.b_half, bx0, by0);
       const int ct = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
       if (a.masks == nullptr) {
-        gate = 0xffffffffu;
+        gate[0] = gate[1] = gate[2] = gate[3] = 0xffffffffu;
       } else if (UNARY) {
         // UnaryUpdateManager.doUpdates gates on the top mask of the cell (CLParser.scala:747).
         const int ctop = cell_abs(a.cell_offsets, sent, n, 1, b, e);
-        gate = mask_any(a.masks + (size_t)ctop * a.mask_words, a.mask_words) ? 0xffffffffu : 0u;
+        if (mask_any(a.masks + (size_t)ctop * a.mask_words, a.mask_words)) gate[0] = 1u;
       } else {
         // Children must be allowed spans (CLParser.scala:892-893) and the target cell's mask
         // must intersect the block mask (CLParser.scala:845-855,867).
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
           for (int bb = 0; bb < a.num_blocks; ++bb) {
             uint32_t hit = 0;
             for (int w = 0; w < a.mask_words; ++w) hit |= tm[w] & a.block_masks[bb * a.mask_words + w];
-            if (hit) gate |= 1u << bb;
+            if (hit) gate[bb >> 5] |= 1u << (bb & 31);
           }
         }
       }
@@ -154,28 +154,39 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     }
     s_cell_a[tid] = ca;
     s_cell_b[tid] = cb;
-    s_gate[tid] = gate;
+    for (int w = 0; w < 4; ++w) s_gate[w][tid] = gate[w];
     s_scale[tid] = scale;
   }
   __syncthreads();
 
-  // Stage operand rows: consecutive threads read consecutive symbols of one cell.
-  for (int idx = tid; idx < rows_a * ITEMS; idx += kBlockThreads) {
-    const int it = idx / rows_a;
-    const int row = idx - it * rows_a;
-    float v = zero;
-    if (tile0 + it < a.num_items) v = a.chA[(size_t)s_cell_a[it] * a.cell_size + a_lo + row];
-    lds_a[row * a.row_stride + it] = v;
-  }
-  if constexpr (!UNARY) {
-    for (int idx = tid; idx < rows_b * ITEMS; idx += kBlockThreads) {
-      const int it = idx / rows_b;
-      const int row = idx - it * rows_b;
-      float v = zero;
-      if (tile0 + it < a.num_items) v = a.chB[(size_t)s_cell_b[it] * a.cell_size + b_lo + row];
-      lds_b[row * a.row_stride + it] = v;
+  // Stage operand rows: consecutive threads read consecutive symbols of one cell; eight loads are
+  // in flight per thread before the LDS stores.
+  auto stage = [&](const float* ch, const int32_t* cells, int lo, int rows, float* dst) {
+    const int total = rows * ITEMS;
+    for (int base = tid; base < total; base += 8 * kBlockThreads) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int idx = base + k * kBlockThreads;
+        v[k] = zero;
+        if (idx < total) {
+          const int it = idx / rows;
+          const int row = idx - it * rows;
+          if (tile0 + it < a.num_items) v[k] = ch[(size_t)cells[it] * a.cell_size + lo + row];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int idx = base + k * kBlockThreads;
+        if (idx < total) {
+          const int it = idx / rows;
+          dst[(idx - it * rows) * a.row_stride + it] = v[k];
+        }
+      }
     }
-  }
+  };
+  stage(a.chA, s_cell_a, a_lo, rows_a, lds_a);
+  if constexpr (!UNARY) stage(a.chB, s_cell_b, b_lo, rows_b, lds_b);
   __syncthreads();
 
   const int lane = tid & 63;
@@ -183,53 +194,92 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   const int nslots = a.chunk_slot_start[chunk + 1] - slot0;
   const char* lane_base = reinterpret_cast<const char*>(smem) + lane * IPL * 4;
   const int kz = (lane & a.kzero) * 16;  // zero at run time; keeps rule loads on the vector path
+  constexpr int U = kRuleUnroll;          // slot rule lists are padded to a multiple of U
   while (true) {
     int s = 0;
     if (lane == 0) s = atomicAdd(&s_next, 1);
-    s = __shfl(s, 0);
+    s = __builtin_amdgcn_readfirstlane(__shfl(s, 0));
     if (s >= nslots) break;
     const int slot = slot0 + s;
-    const int t = a.slot_target[slot];
-    const int rs = a.slot_rule_start[slot];
-    const int rc = a.slot_rule_count[slot];
+    const int t = __builtin_amdgcn_readfirstlane(a.slot_target[slot]);
+    const int rs = __builtin_amdgcn_readfirstlane(a.slot_rule_start[slot]);
+    const int rc = __builtin_amdgcn_readfirstlane(a.slot_rule_count[slot]);
     __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(a.rules + rs), (short)0, rc * 16, 0x00020000);
     float acc[IPL];
 #pragma unroll
     for (int i = 0; i < IPL; ++i) acc[i] = zero;
-#pragma unroll 4
-    for (int r = 0; r < rc; ++r) {
-      const uint4 ru = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, kz, r * 16, 0));
-      const float q = __uint_as_float(ru.z);
-      if (IPL == 2) {
-        const float2 va = *reinterpret_cast<const float2*>(lane_base + ru.x);
-        if (UNARY) {
-          if (SEMI == kViterbi) {
-            acc[0] = fmaxf(acc[0], va.x + q);
-            acc[IPL - 1] = fmaxf(acc[IPL - 1], va.y + q);
+    // Software pipeline with two register sets: the next group's rule loads are in flight while
+    // this group computes. Loads past the slot end read zeros (buffer bounds) and are not used.
+    uint4 ga[U], gb[U];
+    auto load_group = [&](uint4* dst, int r) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        dst[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, kz, (r + u) * 16, 0));
+    };
+    auto compute_group = [&](const uint4* cur) {
+      if constexpr (IPL == 2) {
+        float2 va[U], vb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          va[u] = *reinterpret_cast<const float2*>(lane_base + cur[u].x);
+          if constexpr (!UNARY) vb[u] = *reinterpret_cast<const float2*>(lane_base + cur[u].y);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const float q = __uint_as_float(cur[u].z);
+          if constexpr (UNARY) {
+            if constexpr (SEMI == kViterbi) {
+              acc[0] = fmaxf(acc[0], va[u].x + q);
+              acc[IPL - 1] = fmaxf(acc[IPL - 1], va[u].y + q);
+            } else {
+              acc[0] = fmaf(va[u].x, q, acc[0]);
+              acc[IPL - 1] = fmaf(va[u].y, q, acc[IPL - 1]);
+            }
           } else {
-            acc[0] = fmaf(va.x, q, acc[0]);
-            acc[IPL - 1] = fmaf(va.y, q, acc[IPL - 1]);
-          }
-        } else {
-          const float2 vb = *reinterpret_cast<const float2*>(lane_base + ru.y);
-          if (SEMI == kViterbi) {
-            acc[0] = fmaxf(acc[0], (va.x + vb.x) + q);
-            acc[IPL - 1] = fmaxf(acc[IPL - 1], (va.y + vb.y) + q);
-          } else {
-            acc[0] = fmaf(va.x * vb.x, q, acc[0]);
-            acc[IPL - 1] = fmaf(va.y * vb.y, q, acc[IPL - 1]);
+            if constexpr (SEMI == kViterbi) {
+              acc[0] = fmaxf(acc[0], (va[u].x + vb[u].x) + q);
+              acc[IPL - 1] = fmaxf(acc[IPL - 1], (va[u].y + vb[u].y) + q);
+            } else {
+              acc[0] = fmaf(va[u].x * vb[u].x, q, acc[0]);
+              acc[IPL - 1] = fmaf(va[u].y * vb[u].y, q, acc[IPL - 1]);
+            }
           }
         }
       } else {
-        const float va = *reinterpret_cast<const float*>(lane_base + ru.x);
-        if (UNARY) {
-          acc[0] = SEMI == kViterbi ? fmaxf(acc[0], va + q) : fmaf(va, q, acc[0]);
-        } else {
-          const float vb = *reinterpret_cast<const float*>(lane_base + ru.y);
-          acc[0] = SEMI == kViterbi ? fmaxf(acc[0], (va + vb) + q) : fmaf(va * vb, q, acc[0]);
+        float va[U], vb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          va[u] = *reinterpret_cast<const float*>(lane_base + cur[u].x);
+          if constexpr (!UNARY) vb[u] = *reinterpret_cast<const float*>(lane_base + cur[u].y);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const float q = __uint_as_float(cur[u].z);
+          if constexpr (UNARY) {
+            acc[0] = SEMI == kViterbi ? fmaxf(acc[0], va[u] + q) : fmaf(va[u], q, acc[0]);
+          } else {
+            acc[0] = SEMI == kViterbi ? fmaxf(acc[0], (va[u] + vb[u]) + q) : fmaf(va[u] * vb[u], q, acc[0]);
+          }
         }
       }
+    };
+    // sched_barrier keeps the compiler from hoisting the prefetch above the previous group's
+    // compute, so only the older group's loads are waited for.
+    load_group(ga, 0);
+    for (int r = 0;;) {
+      load_group(gb, r + U);
+      __builtin_amdgcn_sched_barrier(0);
+      compute_group(ga);
+      __builtin_amdgcn_sched_barrier(0);
+      r += U;
+      if (r >= rc) break;
+      load_group(ga, r + U);
+      __builtin_amdgcn_sched_barrier(0);
+      compute_group(gb);
+      __builtin_amdgcn_sched_barrier(0);
+      r += U;
+      if (r >= rc) break;
     }
     const int bit = (UNARY || a.block_of_target == nullptr) ? 0 : a.block_of_target[t];
     float* out = a.partial + (size_t)slot * a.pitch;
@@ -237,7 +287,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     for (int i = 0; i < IPL; ++i) {
       const int li = lane * IPL + i;
       const int item = tile0 + li;
-      float v = ((s_gate[li] >> bit) & 1u) ? acc[i] : zero;
+      float v = ((s_gate[bit >> 5][li] >> (bit & 31)) & 1u) ? acc[i] : zero;
       if (SEMI == kReal) v *= s_scale[li];
       if (item < a.num_items) out[item] = v;
     }
@@ -747,13 +797,21 @@ void Layer::build_family(int32_t id) {
         while (j < cr.size() && cr[j].t == cr[i].t) ++j;
         F.h_slot_target.push_back(cr[i].t);
         slot_rule_start.push_back((int32_t)packed.size());
-        slot_rule_count.push_back((int32_t)(j - i));
+        const size_t padded = (j - i + kRuleUnroll - 1) / kRuleUnroll * kRuleUnroll;
+        slot_rule_count.push_back((int32_t)padded);
         for (size_t k = i; k < j; ++k) {
           uint4 u;
           u.x = (uint32_t)((cr[k].a - a_lo) * stride * 4);
           u.y = F.unary ? 0u : (uint32_t)((rows_a + cr[k].b - b_lo) * stride * 4);
           std::memcpy(&u.z, &cr[k].q, 4);
           u.w = 0;
+          packed.push_back(u);
+        }
+        // Null rules up to the unroll width: weight = semiring zero, so they change nothing.
+        const float null_q = g.semiring == kViterbi ? -INFINITY : 0.0f;
+        for (size_t k = j - i; k < padded; ++k) {
+          uint4 u{0u, 0u, 0u, 0u};
+          std::memcpy(&u.z, &null_q, 4);
           packed.push_back(u);
         }
         i = j;
@@ -785,7 +843,7 @@ void Layer::build_family(int32_t id) {
       F.num_blocks = bp->num_blocks > 0 ? 1 : 0;
       masks.assign(bp->masks.begin(), bp->masks.begin() + std::min<size_t>(bp->masks.size(), g.mask_words));
     }
-    if (F.num_blocks > 32) throw std::runtime_error("more than 32 rule blocks in one family");
+    if (F.num_blocks > 128) throw std::runtime_error("more than 128 rule blocks in one family");
     masks.resize(std::max<size_t>(masks.size(), 1), 0u);
     F.d_block_of_target = upload(bot);
     F.d_block_masks = upload(masks);

@@ -10,6 +10,7 @@ namespace puck {
 constexpr int kMaxGroupFamilies = 8;
 constexpr int kBlockThreads = 1024;   // rule kernels: 16 waves share one LDS item tile
 constexpr int kLdsBudgetBytes = 150 * 1024;
+constexpr int kRuleUnroll = 4;        // rules per pipelined group; slot rule lists are padded to it
 
 // Arguments of the rule-family kernel (one launch = one family, one span, one cell slice).
 struct FamArgs {

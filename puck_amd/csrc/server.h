@@ -1,0 +1,77 @@
+// Continuous-batching parse server over the CKY engine.
+//
+// Replaces the reference's BatchFunctionAnnotatorService (AnnotatorService.scala:29-78), which
+// accumulates requests and runs one CLParser.parse per flush (20,000 sentences or 1 s). Here
+// requests are admitted into batches as soon as one of `max_inflight` batch slots is free; each slot
+// owns a Parser with its own HIP stream and workspace, so batches overlap on the device, and a
+// slot admits requests while their chart cells fit its memory budget.
+#pragma once
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+
+namespace puck {
+
+struct ServerStats {
+  int64_t completed = 0;
+  int64_t batches = 0;
+  double mean_batch = 0;
+  double p50_ms = 0, p90_ms = 0, p99_ms = 0, max_ms = 0;
+};
+
+class Server {
+ public:
+  Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
+         int32_t max_inflight, int32_t max_batch, int32_t items_per_lane);
+  ~Server();
+
+  int64_t submit(const int32_t* words, int32_t len);
+  // Returns true and fills `nodes` when the request is done (status < 0: the parse failed, see
+  // `error`). With wait = true blocks until done.
+  bool result(int64_t ticket, bool wait, std::vector<TreeNode>* nodes, std::vector<MbrNode>* mbr,
+              int32_t* status, std::string* error);
+  void flush();
+  ServerStats stats();
+  void reset_stats();
+
+ private:
+  struct Request {
+    int64_t ticket;
+    std::vector<int32_t> words;
+    std::chrono::steady_clock::time_point submitted;
+  };
+  struct Done {
+    std::vector<TreeNode> nodes;
+    std::vector<MbrNode> mbr;
+    int32_t status = 0;
+    std::string error;
+  };
+  void worker(int32_t slot);
+
+  std::vector<std::unique_ptr<Parser>> parsers_;
+  std::vector<std::thread> threads_;
+  int32_t max_batch_;
+  int64_t max_cells_;
+  bool mbr_final_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<Request> queue_;
+  std::unordered_map<int64_t, Done> done_;
+  int64_t next_ticket_ = 1;
+  int64_t outstanding_ = 0;
+  bool stop_ = false;
+  std::vector<double> latencies_ms_;
+  int64_t batches_ = 0, batched_requests_ = 0;
+};
+
+}  // namespace puck

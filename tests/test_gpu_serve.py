@@ -1,0 +1,47 @@
+"""Serving: results through the continuous-batching server equal the result of parsing each
+sentence alone, whatever mix of requests the server batched it with (Viterbi: bit-exact)."""
+import threading
+
+import numpy as np
+import pytest
+
+from conftest import split_sentences
+
+pytestmark = pytest.mark.gpu
+
+
+def test_server_result_same_alone_and_in_the_mix(puck, wsj1):
+    words, offsets = wsj1.sample(seed=77, n=120, mode=0, min_len=1, max_len=30)
+    sents = split_sentences(words, offsets)
+    alone = puck.CLParser([wsj1], mem_bytes=1 << 30)
+    want = [alone.parse([s])[0] for s in sents[:12]] + alone.parse((words, offsets))[12:]
+    srv = puck.Server([wsj1], mem_bytes=2 << 30, max_inflight=3, max_batch=16)
+    tickets = [None] * len(sents)
+
+    def producer(lo, hi):
+        for i in range(lo, hi):
+            tickets[i] = srv.submit(sents[i])
+
+    threads = [threading.Thread(target=producer, args=(k * 40, (k + 1) * 40)) for k in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for i, s in enumerate(sents):
+        got = srv.result(tickets[i], len(s))
+        assert np.array_equal(got, want[i]), f"request {i}"
+    st = srv.stats()
+    assert st["completed"] == len(sents)
+    assert st["batches"] > 1 and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_server_coarse_to_fine(puck, wsj0, wsj1):
+    words, offsets = wsj1.sample(seed=78, n=30, mode=0, min_len=3, max_len=20)
+    alone = puck.CLParser([wsj0, wsj1], mem_bytes=1 << 30)
+    want = alone.parse((words, offsets))
+    srv = puck.Server([wsj0, wsj1], mem_bytes=2 << 30, max_inflight=2, max_batch=7)
+    sents = split_sentences(words, offsets)
+    tickets = [srv.submit(s) for s in sents]
+    srv.flush()
+    for i, s in enumerate(sents):
+        assert np.array_equal(srv.result(tickets[i], len(s), wait=False), want[i])
