@@ -62,11 +62,9 @@ struct DevFamily {
   std::vector<int32_t> h_chunk_slot_start;  // num_chunks + 1
   std::vector<int32_t> h_slot_target;
   int32_t* d_chunk_range = nullptr;
-  int32_t* d_chunk_slot_start = nullptr;
   int32_t* d_slot_target = nullptr;
-  int32_t* d_slot_rule_start = nullptr;
-  int32_t* d_slot_rule_count = nullptr;
-  uint4* d_rules = nullptr;  // {A row byte offset, B row byte offset, bits(q), 0}
+  int4* d_wave_work = nullptr;  // per (chunk, wave): {first slot, first rule, end rule, 0}
+  uint4* d_rules = nullptr;  // {A row byte offset, B row byte offset, bits(q), end flag | block}
   // Block gating (CLParser.scala:783-785,845-855): block of each target symbol and block masks.
   int32_t num_blocks = 0;
   int32_t* d_block_of_target = nullptr;

@@ -76,7 +76,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   __shared__ int32_t s_cell_a[ITEMS], s_cell_b[ITEMS];
   __shared__ uint32_t s_gate[4][ITEMS];  // 128 block bits per item
   __shared__ float s_scale[ITEMS];
-  __shared__ int32_t s_next;
   const float zero = semi_zero<SEMI>();
   const int tid = threadIdx.x;
   const int tile0 = blockIdx.x * ITEMS;
@@ -88,7 +87,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   float* lds_a = smem;
   float* lds_b = smem + rows_a * a.row_stride;
 
-  if (tid == 0) s_next = 0;
   if (tid < ITEMS) {
     const int item = tile0 + tid;
     int ca = 0, cb = 0;
@@ -190,107 +188,115 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   __syncthreads();
 
   const int lane = tid & 63;
-  const int slot0 = a.chunk_slot_start[chunk];
-  const int nslots = a.chunk_slot_start[chunk + 1] - slot0;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const char* lane_base = reinterpret_cast<const char*>(smem) + lane * IPL * 4;
-  const int kz = (lane & a.kzero) * 16;  // zero at run time; keeps rule loads on the vector path
-  constexpr int U = kRuleUnroll;          // slot rule lists are padded to a multiple of U
-  while (true) {
-    int s = 0;
-    if (lane == 0) s = atomicAdd(&s_next, 1);
-    s = __builtin_amdgcn_readfirstlane(__shfl(s, 0));
-    if (s >= nslots) break;
-    const int slot = slot0 + s;
-    const int t = __builtin_amdgcn_readfirstlane(a.slot_target[slot]);
-    const int rs = __builtin_amdgcn_readfirstlane(a.slot_rule_start[slot]);
-    const int rc = __builtin_amdgcn_readfirstlane(a.slot_rule_count[slot]);
-    __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(a.rules + rs), (short)0, rc * 16, 0x00020000);
-    float acc[IPL];
+  constexpr int U = kRuleUnroll;
+  constexpr int kWaves = kBlockThreads / 64;
+  // Rule records are wave-uniform and never written while a kernel runs: reading them through the
+  // constant address space lets the compiler use scalar loads into SGPRs (16 bytes per rule for
+  // the whole wave, where a vector load would return them to every lane).
+  typedef uint32_t Word4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) Word4 ConstWord4;
+  ConstWord4* rules = (ConstWord4*)a.rules;
+  const Word4 work = ((ConstWord4*)a.wave_work)[chunk * kWaves + wave];
+  int slot = (int)work.x;
+  int r = (int)work.y;
+  const int r_end = (int)work.z;
+  if (r >= r_end) return;
+
+  // Per-item gate words and scale, kept in registers for the slot flushes.
+  uint64_t gate_lo[IPL], gate_hi[IPL];  // block bits 0-63 and 64-127
+  float scl[IPL];
 #pragma unroll
-    for (int i = 0; i < IPL; ++i) acc[i] = zero;
-    // Software pipeline with two register sets: the next group's rule loads are in flight while
-    // this group computes. Loads past the slot end read zeros (buffer bounds) and are not used.
-    uint4 ga[U], gb[U];
-    auto load_group = [&](uint4* dst, int r) {
+  for (int i = 0; i < IPL; ++i) {
+    const int li = lane * IPL + i;
+    gate_lo[i] = (uint64_t)s_gate[0][li] | ((uint64_t)s_gate[1][li] << 32);
+    gate_hi[i] = (uint64_t)s_gate[2][li] | ((uint64_t)s_gate[3][li] << 32);
+    scl[i] = s_scale[li];
+  }
+  auto read_group = [&](const Word4* g, float (*va)[IPL], float (*vb)[IPL]) {
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        dst[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, kz, (r + u) * 16, 0));
-    };
-    auto compute_group = [&](const uint4* cur) {
+    for (int u = 0; u < U; ++u) {
       if constexpr (IPL == 2) {
-        float2 va[U], vb[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          va[u] = *reinterpret_cast<const float2*>(lane_base + cur[u].x);
-          if constexpr (!UNARY) vb[u] = *reinterpret_cast<const float2*>(lane_base + cur[u].y);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const float q = __uint_as_float(cur[u].z);
-          if constexpr (UNARY) {
-            if constexpr (SEMI == kViterbi) {
-              acc[0] = fmaxf(acc[0], va[u].x + q);
-              acc[IPL - 1] = fmaxf(acc[IPL - 1], va[u].y + q);
-            } else {
-              acc[0] = fmaf(va[u].x, q, acc[0]);
-              acc[IPL - 1] = fmaf(va[u].y, q, acc[IPL - 1]);
-            }
-          } else {
-            if constexpr (SEMI == kViterbi) {
-              acc[0] = fmaxf(acc[0], (va[u].x + vb[u].x) + q);
-              acc[IPL - 1] = fmaxf(acc[IPL - 1], (va[u].y + vb[u].y) + q);
-            } else {
-              acc[0] = fmaf(va[u].x * vb[u].x, q, acc[0]);
-              acc[IPL - 1] = fmaf(va[u].y * vb[u].y, q, acc[IPL - 1]);
-            }
-          }
+        const float2 x = *reinterpret_cast<const float2*>(lane_base + g[u].x);
+        va[u][0] = x.x;
+        va[u][IPL - 1] = x.y;
+        if constexpr (!UNARY) {
+          const float2 y = *reinterpret_cast<const float2*>(lane_base + g[u].y);
+          vb[u][0] = y.x;
+          vb[u][IPL - 1] = y.y;
         }
       } else {
-        float va[U], vb[U];
+        va[u][0] = *reinterpret_cast<const float*>(lane_base + g[u].x);
+        if constexpr (!UNARY) vb[u][0] = *reinterpret_cast<const float*>(lane_base + g[u].y);
+      }
+    }
+  };
+  float acc[IPL];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          va[u] = *reinterpret_cast<const float*>(lane_base + cur[u].x);
-          if constexpr (!UNARY) vb[u] = *reinterpret_cast<const float*>(lane_base + cur[u].y);
-        }
+  for (int i = 0; i < IPL; ++i) acc[i] = zero;
+  // Software pipeline over groups of U rules: while group g is computed, the LDS operand reads of
+  // group g+1 and the scalar rule loads of group g+2 are in flight. Scalar loads share the LDS wait
+  // counter, so the single wait at the top of an iteration covers both. Records past r_end belong
+  // to the next wave or to the null tail of the array: they are read but never used.
+  Word4 cur[U], nxt[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const float q = __uint_as_float(cur[u].z);
-          if constexpr (UNARY) {
-            acc[0] = SEMI == kViterbi ? fmaxf(acc[0], va[u] + q) : fmaf(va[u], q, acc[0]);
-          } else {
-            acc[0] = SEMI == kViterbi ? fmaxf(acc[0], (va[u] + vb[u]) + q) : fmaf(va[u] * vb[u], q, acc[0]);
-          }
+  for (int u = 0; u < U; ++u) {
+    cur[u] = rules[r + u];
+    nxt[u] = rules[r + U + u];
+  }
+  // One pipeline step: issue the reads of the next group into (la, lb), compute the current group
+  // from (da, db). Two operand register sets alternate between steps, so nothing is copied.
+  auto step = [&](float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
+    read_group(nxt, la, lb);
+    Word4 nn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) nn[u] = rules[r + 2 * U + u];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float q = __uint_as_float(cur[u].z);
+#pragma unroll
+      for (int i = 0; i < IPL; ++i) {
+        if constexpr (UNARY) {
+          acc[i] = SEMI == kViterbi ? fmaxf(acc[i], da[u][i] + q) : fmaf(da[u][i], q, acc[i]);
+        } else {
+          acc[i] = SEMI == kViterbi ? fmaxf(acc[i], (da[u][i] + db[u][i]) + q)
+                                    : fmaf(da[u][i] * db[u][i], q, acc[i]);
         }
       }
-    };
-    // sched_barrier keeps the compiler from hoisting the prefetch above the previous group's
-    // compute, so only the older group's loads are waited for.
-    load_group(ga, 0);
-    for (int r = 0;;) {
-      load_group(gb, r + U);
-      __builtin_amdgcn_sched_barrier(0);
-      compute_group(ga);
-      __builtin_amdgcn_sched_barrier(0);
-      r += U;
-      if (r >= rc) break;
-      load_group(ga, r + U);
-      __builtin_amdgcn_sched_barrier(0);
-      compute_group(gb);
-      __builtin_amdgcn_sched_barrier(0);
-      r += U;
-      if (r >= rc) break;
     }
-    const int bit = (UNARY || a.block_of_target == nullptr) ? 0 : a.block_of_target[t];
-    float* out = a.partial + (size_t)slot * a.pitch;
+    const uint32_t tag = cur[U - 1].w;
+    if (tag & kSlotEnd) {
+      // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
+      const int bit = (int)(tag & 127u);
+      float* out = a.partial + (size_t)slot * a.pitch;
 #pragma unroll
-    for (int i = 0; i < IPL; ++i) {
-      const int li = lane * IPL + i;
-      const int item = tile0 + li;
-      float v = ((s_gate[bit >> 5][li] >> (bit & 31)) & 1u) ? acc[i] : zero;
-      if (SEMI == kReal) v *= s_scale[li];
-      if (item < a.num_items) out[item] = v;
+      for (int i = 0; i < IPL; ++i) {
+        const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
+        float v = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
+        if (SEMI == kReal) v *= scl[i];
+        const int item = tile0 + lane * IPL + i;
+        if (item < a.num_items) out[item] = v;
+        acc[i] = zero;
+      }
+      ++slot;
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cur[u] = nxt[u];
+      nxt[u] = nn[u];
+    }
+    r += U;
+  };
+  float a0[U][IPL], b0[U][IPL], a1[U][IPL], b1[U][IPL];
+  read_group(cur, a0, b0);
+  for (;;) {
+    step(a0, b0, a1, b1);
+    if (r >= r_end) break;
+    step(a1, b1, a0, b0);
+    if (r >= r_end) break;
   }
 }
 
@@ -753,9 +759,10 @@ void Layer::build_family(int32_t id) {
   F.num_rules = (int32_t)rules.size();
   F.rule_apps_per_item = (double)rules.size();
 
-  // Chunking of the operand symbol spaces so that one tile's rows fit the LDS budget.
+  // Chunking of the operand symbol spaces so that one tile's rows fit the LDS budget (one
+  // workgroup per CU with two items per lane, two workgroups per CU with one).
   const int stride = 64 * ipl + (ipl == 1 ? 1 : 2);
-  const int max_rows = kLdsBudgetBytes / (stride * 4);
+  const int max_rows = (ipl == 1 ? kLdsBudgetBytes / 2 : kLdsBudgetBytes) / (stride * 4);
   int ca = 1, cb = 1;
   if (F.unary) {
     ca = (F.n_a + max_rows - 1) / max_rows;
@@ -775,62 +782,10 @@ void Layer::build_family(int32_t id) {
   const int ra = (std::max(1, F.n_a) + ca - 1) / ca;
   const int rb = F.unary ? 0 : (std::max(1, F.n_b) + cb - 1) / cb;
   F.max_rows = ra + rb;
-  std::vector<int32_t> slot_rule_start, slot_rule_count;
-  std::vector<uint4> packed;
-  F.h_chunk_slot_start.push_back(0);
-  for (int ia = 0; ia < ca; ++ia)
-    for (int ib = 0; ib < (F.unary ? 1 : cb); ++ib) {
-      const int a_lo = ia * ra, a_hi = std::min(F.n_a, a_lo + ra);
-      const int b_lo = F.unary ? 0 : ib * rb, b_hi = F.unary ? 0 : std::min(F.n_b, b_lo + rb);
-      std::vector<R4> cr;
-      for (auto& r : rules)
-        if (r.a >= a_lo && r.a < a_hi && (F.unary || (r.b >= b_lo && r.b < b_hi))) cr.push_back(r);
-      std::stable_sort(cr.begin(), cr.end(), [](const R4& x, const R4& y) {
-        if (x.t != y.t) return x.t < y.t;
-        if (x.a != y.a) return x.a < y.a;
-        return x.b < y.b;
-      });
-      F.h_chunk_range.insert(F.h_chunk_range.end(), {a_lo, a_hi, b_lo, b_hi});
-      const int rows_a = a_hi - a_lo;
-      for (size_t i = 0; i < cr.size();) {
-        size_t j = i;
-        while (j < cr.size() && cr[j].t == cr[i].t) ++j;
-        F.h_slot_target.push_back(cr[i].t);
-        slot_rule_start.push_back((int32_t)packed.size());
-        const size_t padded = (j - i + kRuleUnroll - 1) / kRuleUnroll * kRuleUnroll;
-        slot_rule_count.push_back((int32_t)padded);
-        for (size_t k = i; k < j; ++k) {
-          uint4 u;
-          u.x = (uint32_t)((cr[k].a - a_lo) * stride * 4);
-          u.y = F.unary ? 0u : (uint32_t)((rows_a + cr[k].b - b_lo) * stride * 4);
-          std::memcpy(&u.z, &cr[k].q, 4);
-          u.w = 0;
-          packed.push_back(u);
-        }
-        // Null rules up to the unroll width: weight = semiring zero, so they change nothing.
-        const float null_q = g.semiring == kViterbi ? -INFINITY : 0.0f;
-        for (size_t k = j - i; k < padded; ++k) {
-          uint4 u{0u, 0u, 0u, 0u};
-          std::memcpy(&u.z, &null_q, 4);
-          packed.push_back(u);
-        }
-        i = j;
-      }
-      F.h_chunk_slot_start.push_back((int32_t)F.h_slot_target.size());
-      ++F.num_chunks;
-    }
-  F.num_slots = (int32_t)F.h_slot_target.size();
-  F.d_chunk_range = upload(F.h_chunk_range);
-  F.d_chunk_slot_start = upload(F.h_chunk_slot_start);
-  F.d_slot_target = upload(F.h_slot_target);
-  F.d_slot_rule_start = upload(slot_rule_start);
-  F.d_slot_rule_count = upload(slot_rule_count);
-  F.d_rules = upload(packed);
-
+  // Block gating tables (binary families): the block bit of each target symbol. Non-final layers
+  // merge all kernels into one block whose mask is the first kernel's (CLParser.scala:783-785).
+  std::vector<int32_t> bot(F.n_t, 0);
   if (!F.unary) {
-    // Block gating tables. Non-final layers merge all kernels into one block whose mask is the
-    // first kernel's (CLParser.scala:783-785).
-    std::vector<int32_t> bot(F.n_t, 0);
     std::vector<uint32_t> masks;
     if (final_layer) {
       F.num_blocks = bp->num_blocks;
@@ -848,6 +803,88 @@ void Layer::build_family(int32_t id) {
     F.d_block_of_target = upload(bot);
     F.d_block_masks = upload(masks);
   }
+
+  // Rule stream. Per chunk, the target symbols (slots) are assigned to the workgroup's waves by
+  // longest-processing-time-first over their padded rule counts; each wave's slots are laid out
+  // back to back so that the wave streams one contiguous rule range. Slot ids follow that layout.
+  // A slot's rules are padded with null rules (semiring-zero weight) to a multiple of the unroll
+  // width, and the last record of the slot carries kSlotEnd | block bit in its fourth word.
+  constexpr int kWaves = kBlockThreads / 64;
+  std::vector<uint4> packed;
+  std::vector<int4> wave_work;
+  const float null_q = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  F.h_chunk_slot_start.push_back(0);
+  for (int ia = 0; ia < ca; ++ia)
+    for (int ib = 0; ib < (F.unary ? 1 : cb); ++ib) {
+      const int a_lo = ia * ra, a_hi = std::min(F.n_a, a_lo + ra);
+      const int b_lo = F.unary ? 0 : ib * rb, b_hi = F.unary ? 0 : std::min(F.n_b, b_lo + rb);
+      std::vector<R4> cr;
+      for (auto& r : rules)
+        if (r.a >= a_lo && r.a < a_hi && (F.unary || (r.b >= b_lo && r.b < b_hi))) cr.push_back(r);
+      std::stable_sort(cr.begin(), cr.end(), [](const R4& x, const R4& y) {
+        if (x.t != y.t) return x.t < y.t;
+        if (x.a != y.a) return x.a < y.a;
+        return x.b < y.b;
+      });
+      F.h_chunk_range.insert(F.h_chunk_range.end(), {a_lo, a_hi, b_lo, b_hi});
+      const int rows_a = a_hi - a_lo;
+      // Slots of this chunk: [first, last) ranges of cr with one target each.
+      std::vector<std::pair<size_t, size_t>> slots;
+      for (size_t i = 0; i < cr.size();) {
+        size_t j = i;
+        while (j < cr.size() && cr[j].t == cr[i].t) ++j;
+        slots.push_back({i, j});
+        i = j;
+      }
+      auto padded = [](size_t n) { return (n + kRuleUnroll - 1) / kRuleUnroll * kRuleUnroll; };
+      std::vector<size_t> order(slots.size());
+      for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+      std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return slots[x].second - slots[x].first > slots[y].second - slots[y].first;
+      });
+      // Per slot the flush costs about two rule groups; count it in the balance.
+      std::vector<int64_t> load(kWaves, 0);
+      std::vector<std::vector<size_t>> of_wave(kWaves);
+      for (size_t k : order) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[w] += (int64_t)padded(slots[k].second - slots[k].first) + 2 * kRuleUnroll;
+        of_wave[w].push_back(k);
+      }
+      for (int w = 0; w < kWaves; ++w) {
+        int4 ww{(int32_t)F.h_slot_target.size(), (int32_t)packed.size(), 0, 0};
+        for (size_t k : of_wave[w]) {
+          const size_t i = slots[k].first, j = slots[k].second;
+          const int t = cr[i].t;
+          F.h_slot_target.push_back(t);
+          for (size_t m = i; m < j; ++m) {
+            uint4 u;
+            u.x = (uint32_t)((cr[m].a - a_lo) * stride * 4);
+            u.y = F.unary ? 0u : (uint32_t)((rows_a + cr[m].b - b_lo) * stride * 4);
+            std::memcpy(&u.z, &cr[m].q, 4);
+            u.w = 0;
+            packed.push_back(u);
+          }
+          for (size_t m = j - i; m < padded(j - i); ++m) {
+            uint4 u{0u, 0u, 0u, 0u};
+            std::memcpy(&u.z, &null_q, 4);
+            packed.push_back(u);
+          }
+          packed.back().w = kSlotEnd | (uint32_t)bot[t];
+        }
+        ww.z = (int32_t)packed.size();
+        wave_work.push_back(ww);
+      }
+      F.h_chunk_slot_start.push_back((int32_t)F.h_slot_target.size());
+      ++F.num_chunks;
+    }
+  // The rule loop reads two groups past the end of a wave's range: a tail of null records keeps
+  // those reads inside the array (their contents are never used).
+  for (int k = 0; k < 2 * kRuleUnroll; ++k) packed.push_back(uint4{0u, 0u, 0u, 0u});
+  F.num_slots = (int32_t)F.h_slot_target.size();
+  F.d_chunk_range = upload(F.h_chunk_range);
+  F.d_slot_target = upload(F.h_slot_target);
+  F.d_rules = upload(packed);
+  F.d_wave_work = upload(wave_work);
 }
 
 void Layer::build_reduce(int32_t rid, std::vector<int32_t> fams, int32_t n_t) {
@@ -1029,10 +1066,7 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   if (nitems <= 0) return;
   a.num_items = (int32_t)nitems;
   a.chunk_range = F.d_chunk_range;
-  a.chunk_slot_start = F.d_chunk_slot_start;
-  a.slot_target = F.d_slot_target;
-  a.slot_rule_start = F.d_slot_rule_start;
-  a.slot_rule_count = F.d_slot_rule_count;
+  a.wave_work = F.d_wave_work;
   a.rules = F.d_rules;
   a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
   a.block_masks = F.unary ? nullptr : F.d_block_masks;
@@ -1040,7 +1074,6 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   a.partial = partial;
   const int items_tile = 64 * L.ipl;
   a.pitch = (int32_t)((nitems + 255) / 256 * 256);
-  a.kzero = 0;
   a.span = span;
   a.kind = F.kind;
   a.range_split = F.range_split;

@@ -11,6 +11,7 @@ constexpr int kMaxGroupFamilies = 8;
 constexpr int kBlockThreads = 1024;   // rule kernels: 16 waves share one LDS item tile
 constexpr int kLdsBudgetBytes = 150 * 1024;
 constexpr int kRuleUnroll = 4;        // rules per pipelined group; slot rule lists are padded to it
+constexpr uint32_t kSlotEnd = 0x80000000u;  // flag in the last rule record of a slot
 
 // Arguments of the rule-family kernel (one launch = one family, one span, one cell slice).
 struct FamArgs {
@@ -25,11 +26,8 @@ struct FamArgs {
   const int32_t* cell_list;  // slice cells, packed (sentence << 8) | begin
   const int32_t* prefix;     // slice item prefix (num_cells + 1), null = uniform count
   const int32_t* chunk_range;
-  const int32_t* chunk_slot_start;
-  const int32_t* slot_target;
-  const int32_t* slot_rule_start;
-  const int32_t* slot_rule_count;
-  const uint4* rules;  // {A row byte offset, B row byte offset, bits(q), 0}
+  const int4* wave_work;  // per (chunk, wave): {first slot, first rule, end rule, 0}
+  const uint4* rules;  // {A row byte offset, B row byte offset, bits(q), kSlotEnd | block or 0}
   const int32_t* block_of_target;
   const uint32_t* block_masks;
   float* partial;
@@ -37,7 +35,7 @@ struct FamArgs {
   int32_t span, kind, range_split, cell_size;
   int32_t a_half, b_half, t_half;
   int32_t mask_words, num_blocks, unary_gate;
-  int32_t pitch, kzero;
+  int32_t pitch;
   int32_t row_stride;  // floats per LDS row
 };
 
