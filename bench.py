@@ -73,7 +73,19 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="sentences timed on the CPU oracle (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--check", type=int, default=8, help="sentences whose trees are checked against the oracle")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        n = args.copy_probe << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
               "sentences are independent and would shard by sentence across GPUs.", file=sys.stderr)

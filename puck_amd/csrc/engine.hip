@@ -301,36 +301,84 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
 }
 
 // Combine partials: target[cell][t] = (+) over (family, slot) of t over the cell's items.
+// One workgroup = 64 consecutive cells x kReduceTargets targets, one wave per target. For each
+// partial row feeding its target, a wave streams the items of its 64 cells (one contiguous range)
+// through a per-wave LDS buffer with coalesced loads; each lane then folds its own cell's items.
+// The block's results are transposed through LDS so that each cell's targets are written as one
+// contiguous run of the chart row.
 template <int SEMI>
-__global__ void k_reduce(RedArgs a) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int t = blockIdx.y * 4 + threadIdx.y;
-  if (k >= a.num_cells || t >= a.n_t) return;
+__global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
+  constexpr int CAP = 512;  // floats per wave buffer
+  __shared__ float s_buf[kReduceTargets][CAP];
+  __shared__ float s_out[kReduceTargets][65];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int k0 = blockIdx.x * 64;
+  const int kend = min(k0 + 64, a.num_cells);
+  const int t = blockIdx.y * kReduceTargets + w;
+  const int my_k = k0 + lane;
   float acc = semi_zero<SEMI>();
-  for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
-    const int f = a.fam[e];
-    const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
-    int j0, j1;
-    if (a.prefix[f] == nullptr) {
-      j0 = k * a.uniform_count[f];
-      j1 = j0 + a.uniform_count[f];
-    } else {
-      j0 = a.prefix[f][k] - a.base[f];
-      j1 = a.prefix[f][k + 1] - a.base[f];
-    }
-    for (int j = j0; j < j1; ++j) {
-      if (SEMI == kViterbi) acc = fmaxf(acc, p[j]); else acc += p[j];
+  float* buf = s_buf[w];
+  if (t < a.n_t) {
+    for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
+      const int f = a.fam[e];
+      const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
+      int lo0, hi0, my_lo, my_hi;
+      if (a.prefix[f] == nullptr) {
+        const int uc = a.uniform_count[f];
+        lo0 = k0 * uc;
+        hi0 = kend * uc;
+        my_lo = my_k * uc;
+        my_hi = my_k < kend ? my_lo + uc : my_lo;
+      } else {
+        const int32_t* pre = a.prefix[f];
+        const int base = a.base[f];
+        lo0 = pre[k0] - base;
+        hi0 = pre[kend] - base;
+        my_lo = my_k < kend ? pre[my_k] - base : hi0;
+        my_hi = my_k < kend ? pre[my_k + 1] - base : hi0;
+      }
+      for (int pb = lo0; pb < hi0; pb += CAP) {
+        const int len = min(CAP, hi0 - pb);
+#pragma unroll
+        for (int u = 0; u < CAP / 64; ++u) {
+          const int i = lane + 64 * u;
+          if (i < len) buf[i] = p[pb + i];
+        }
+        // LDS operations of one wave complete in order; the fences keep the compiler from moving
+        // the reads above the writes (or the next piece's writes above these reads).
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int jz = min(my_hi, pb + len);
+        for (int j = max(my_lo, pb); j < jz; ++j) {
+          const float v = buf[j - pb];
+          if (SEMI == kViterbi) acc = fmaxf(acc, v); else acc += v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
   }
+  s_out[w][lane] = acc;
+  __syncthreads();
+  // Write: thread i -> cell i / kReduceTargets, target i % kReduceTargets.
+  const int tt = threadIdx.x % kReduceTargets;
+  const int kk = threadIdx.x / kReduceTargets;
+  const int t_out = blockIdx.y * kReduceTargets + tt;
+  const int k = k0 + kk;
+  if (k >= a.num_cells || t_out >= a.n_t) return;
+  float v = s_out[tt][kk];
   const int packed = a.cell_list[k];
   const int sent = packed >> 8, b = packed & 255, e = b + a.span;
   const int n = a.lengths[sent];
   if (a.root_init && b == 0 && e == n) {
     // setRootScores (CLParserUtils.scala:66-70) on the outside top root cell.
-    acc = (t == a.root) ? a.one : semi_zero<SEMI>();
+    v = (t_out == a.root) ? a.one : semi_zero<SEMI>();
   }
   const int c = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
-  a.chT[(size_t)c * a.cell_size + t] = acc;
+  a.chT[(size_t)c * a.cell_size + t_out] = v;
 }
 
 __global__ void k_fill(float* p, size_t n, float v) {
@@ -1216,10 +1264,10 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     ra.root_init = (rid == kROutBin);
     ra.root = g.root;
     ra.one = g.semiring == kViterbi ? 0.0f : 1.0f;
-    dim3 grid((kc + 63) / 64, (R.n_t + 3) / 4);
+    dim3 grid((kc + 63) / 64, (R.n_t + kReduceTargets - 1) / kReduceTargets);
     launch_semi(g.semiring, [&](auto semi) {
       constexpr int S = decltype(semi)::value;
-      hipLaunchKernelGGL((k_reduce<S>), grid, dim3(64, 4), 0, stream_, ra);
+      hipLaunchKernelGGL((k_reduce<S>), grid, dim3(64 * kReduceTargets), 0, stream_, ra);
     });
     HIP_CHECK(hipGetLastError());
     k0 = k1;

@@ -12,6 +12,7 @@ constexpr int kBlockThreads = 1024;   // rule kernels: 16 waves share one LDS it
 constexpr int kLdsBudgetBytes = 150 * 1024;
 constexpr int kRuleUnroll = 4;        // rules per pipelined group; slot rule lists are padded to it
 constexpr uint32_t kSlotEnd = 0x80000000u;  // flag in the last rule record of a slot
+constexpr int kReduceTargets = 16;    // reduce kernel: targets (waves) per workgroup
 
 // Arguments of the rule-family kernel (one launch = one family, one span, one cell slice).
 struct FamArgs {
