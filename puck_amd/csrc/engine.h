@@ -63,8 +63,8 @@ struct DevFamily {
   std::vector<int32_t> h_slot_target;
   int32_t* d_chunk_range = nullptr;
   int32_t* d_slot_target = nullptr;
-  int4* d_wave_work = nullptr;  // per (chunk, wave): {first slot, first rule, end rule, 0}
-  uint4* d_rules = nullptr;  // {A row byte offset, B row byte offset, bits(q), end flag | block}
+  int4* d_wave_work = nullptr;  // per (chunk, wave): {first slot, first group, end group, 0}
+  uint32_t* d_rules = nullptr;  // rule groups (kGroupWords dwords each)
   // Block gating (CLParser.scala:783-785,845-855): block of each target symbol and block masks.
   int32_t num_blocks = 0;
   int32_t* d_block_of_target = nullptr;

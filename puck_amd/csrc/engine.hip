@@ -190,19 +190,27 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const char* lane_base = reinterpret_cast<const char*>(smem) + lane * IPL * 4;
-  constexpr int U = kRuleUnroll;
+  constexpr int G = kGroupRules;
+  constexpr int H = kRuleUnroll;
   constexpr int kWaves = kBlockThreads / 64;
-  // Rule records are wave-uniform and never written while a kernel runs: reading them through the
-  // constant address space lets the compiler use scalar loads into SGPRs (16 bytes per rule for
-  // the whole wave, where a vector load would return them to every lane).
-  typedef uint32_t Word4 __attribute__((ext_vector_type(4)));
-  typedef const __attribute__((address_space(4))) Word4 ConstWord4;
-  ConstWord4* rules = (ConstWord4*)a.rules;
-  const Word4 work = ((ConstWord4*)a.wave_work)[chunk * kWaves + wave];
+  // Rule groups are wave-uniform and never written while a kernel runs: reading them through the
+  // constant address space lets the compiler use scalar loads into SGPRs (two s_load_dwordx16 per
+  // group of eight rules for the whole wave).
+  typedef uint32_t W16 __attribute__((ext_vector_type(16)));
+  typedef uint32_t W8 __attribute__((ext_vector_type(8)));
+  typedef uint32_t W4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) W16 ConstW16;
+  typedef const __attribute__((address_space(4))) W8 ConstW8;
+  typedef const __attribute__((address_space(4))) W4 ConstW4;
+  typedef const __attribute__((address_space(4))) char ConstByte;
+  ConstByte* gbase = (ConstByte*)a.rules;
+  auto group_off = [&](int g) { return *(ConstW16*)(gbase + (size_t)g * (kGroupWords * 4)); };
+  auto group_q = [&](int g) { return *(ConstW8*)(gbase + (size_t)g * (kGroupWords * 4) + 64); };
+  const W4 work = ((ConstW4*)a.wave_work)[chunk * kWaves + wave];
   int slot = (int)work.x;
-  int r = (int)work.y;
-  const int r_end = (int)work.z;
-  if (r >= r_end) return;
+  int gi = (int)work.y;
+  const int g_end = (int)work.z;
+  if (gi >= g_end) return;
 
   // Per-item gate words and scale, kept in registers for the slot flushes.
   uint64_t gate_lo[IPL], gate_hi[IPL];  // block bits 0-63 and 64-127
@@ -214,48 +222,48 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     gate_hi[i] = (uint64_t)s_gate[2][li] | ((uint64_t)s_gate[3][li] << 32);
     scl[i] = s_scale[li];
   }
-  auto read_group = [&](const Word4* g, float (*va)[IPL], float (*vb)[IPL]) {
+  // Operand reads of one group: word u = A-row byte offset, word G + u = B-row byte offset.
+  auto read_group = [&](const W16& off, float (*va)[IPL], float (*vb)[IPL]) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < G; ++u) {
+      const uint32_t xo = (u % H == H - 1) ? (off[u] & kOffsetMask) : off[u];
       if constexpr (IPL == 2) {
-        const float2 x = *reinterpret_cast<const float2*>(lane_base + g[u].x);
+        const float2 x = *reinterpret_cast<const float2*>(lane_base + xo);
         va[u][0] = x.x;
         va[u][IPL - 1] = x.y;
         if constexpr (!UNARY) {
-          const float2 y = *reinterpret_cast<const float2*>(lane_base + g[u].y);
+          const float2 y = *reinterpret_cast<const float2*>(lane_base + off[G + u]);
           vb[u][0] = y.x;
           vb[u][IPL - 1] = y.y;
         }
       } else {
-        va[u][0] = *reinterpret_cast<const float*>(lane_base + g[u].x);
-        if constexpr (!UNARY) vb[u][0] = *reinterpret_cast<const float*>(lane_base + g[u].y);
+        va[u][0] = *reinterpret_cast<const float*>(lane_base + xo);
+        if constexpr (!UNARY) vb[u][0] = *reinterpret_cast<const float*>(lane_base + off[G + u]);
       }
     }
   };
   float acc[IPL];
 #pragma unroll
   for (int i = 0; i < IPL; ++i) acc[i] = zero;
-  // Software pipeline over groups of U rules: while group g is computed, the LDS operand reads of
-  // group g+1 and the scalar rule loads of group g+2 are in flight. Scalar loads share the LDS wait
-  // counter, so the single wait at the top of an iteration covers both. Records past r_end belong
-  // to the next wave or to the null tail of the array: they are read but never used.
-  Word4 cur[U], nxt[U];
+  auto flush = [&](uint32_t tag) {
+    // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
+    const int bit = (int)((tag >> 24) & 127u);
+    float* out = a.partial + (size_t)slot * a.pitch;
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    cur[u] = rules[r + u];
-    nxt[u] = rules[r + U + u];
-  }
-  // One pipeline step: issue the reads of the next group into (la, lb), compute the current group
-  // from (da, db). Two operand register sets alternate between steps, so nothing is copied.
-  auto step = [&](float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
-    read_group(nxt, la, lb);
-    Word4 nn[U];
+    for (int i = 0; i < IPL; ++i) {
+      const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
+      float v = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
+      if (SEMI == kReal) v *= scl[i];
+      const int item = tile0 + lane * IPL + i;
+      if (item < a.num_items) out[item] = v;
+      acc[i] = zero;
+    }
+    ++slot;
+  };
+  auto half = [&](const W8& qs, uint32_t tag, float (*da)[IPL], float (*db)[IPL], int h) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) nn[u] = rules[r + 2 * U + u];
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float q = __uint_as_float(cur[u].z);
+    for (int u = h * H; u < h * H + H; ++u) {
+      const float q = __uint_as_float(qs[u]);
 #pragma unroll
       for (int i = 0; i < IPL; ++i) {
         if constexpr (UNARY) {
@@ -266,37 +274,39 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
         }
       }
     }
-    const uint32_t tag = cur[U - 1].w;
-    if (tag & kSlotEnd) {
-      // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
-      const int bit = (int)(tag & 127u);
-      float* out = a.partial + (size_t)slot * a.pitch;
-#pragma unroll
-      for (int i = 0; i < IPL; ++i) {
-        const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
-        float v = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
-        if (SEMI == kReal) v *= scl[i];
-        const int item = tile0 + lane * IPL + i;
-        if (item < a.num_items) out[item] = v;
-        acc[i] = zero;
-      }
-      ++slot;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      cur[u] = nxt[u];
-      nxt[u] = nn[u];
-    }
-    r += U;
+    if (tag & kSlotEnd) flush(tag);
   };
-  float a0[U][IPL], b0[U][IPL], a1[U][IPL], b1[U][IPL];
-  read_group(cur, a0, b0);
+  // Software pipeline over groups: while group g is computed from (da, db) with weights cur_q
+  // and tags (cur_t0, cur_t1), the LDS reads of group g+1 (offsets nxt_off) go to (la, lb), the
+  // offsets of group g+2 and the weights of group g+1 are loaded into SGPRs. Scalar loads share
+  // the LDS wait counter, so the one wait the compiler places before the next step's LDS reads
+  // covers both. Register sets alternate between two unrolled steps, so nothing is copied.
+  // Groups past g_end belong to the next wave or to the null tail: read, never used.
+  auto step = [&](const W8& cur_q, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
+                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q,
+                  float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
+    read_group(nxt_off, la, lb);
+    nxt_t0 = nxt_off[H - 1];
+    nxt_t1 = nxt_off[G - 1];
+    nn_off = group_off(gi + 2);
+    nxt_q = group_q(gi + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    half(cur_q, cur_t0, da, db, 0);
+    half(cur_q, cur_t1, da, db, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    ++gi;
+  };
+  W16 oA = group_off(gi), oB = group_off(gi + 1);
+  W8 qA = group_q(gi), qB;
+  uint32_t tA0 = oA[H - 1], tA1 = oA[G - 1], tB0 = 0, tB1 = 0;
+  float a0[G][IPL], b0[G][IPL], a1[G][IPL], b1[G][IPL];
+  read_group(oA, a0, b0);
   for (;;) {
-    step(a0, b0, a1, b1);
-    if (r >= r_end) break;
-    step(a1, b1, a0, b0);
-    if (r >= r_end) break;
+    // cur = A (data 0); reads group g+1 from oB into data 1; loads offsets g+2 into oA.
+    step(qA, tA0, tA1, oB, tB0, tB1, oA, qB, a0, b0, a1, b1);
+    if (gi >= g_end) break;
+    step(qB, tB0, tB1, oA, tA0, tA1, oB, qA, a1, b1, a0, b0);
+    if (gi >= g_end) break;
   }
 }
 
@@ -855,10 +865,11 @@ void Layer::build_family(int32_t id) {
   // Rule stream. Per chunk, the target symbols (slots) are assigned to the workgroup's waves by
   // longest-processing-time-first over their padded rule counts; each wave's slots are laid out
   // back to back so that the wave streams one contiguous rule range. Slot ids follow that layout.
-  // A slot's rules are padded with null rules (semiring-zero weight) to a multiple of the unroll
-  // width, and the last record of the slot carries kSlotEnd | block bit in its fourth word.
+  // A slot's rules are padded with null rules (semiring-zero weight) to a multiple of kRuleUnroll
+  // and the last rule of the slot is tagged kSlotEnd | block bit. Each wave's stream is padded to
+  // whole groups of kGroupRules and stored group-major (see kGroupWords).
   constexpr int kWaves = kBlockThreads / 64;
-  std::vector<uint4> packed;
+  std::vector<uint32_t> packed;  // kGroupWords per group
   std::vector<int4> wave_work;
   const float null_q = g.semiring == kViterbi ? -INFINITY : 0.0f;
   F.h_chunk_slot_start.push_back(0);
@@ -899,7 +910,8 @@ void Layer::build_family(int32_t id) {
         of_wave[w].push_back(k);
       }
       for (int w = 0; w < kWaves; ++w) {
-        int4 ww{(int32_t)F.h_slot_target.size(), (int32_t)packed.size(), 0, 0};
+        int4 ww{(int32_t)F.h_slot_target.size(), (int32_t)(packed.size() / kGroupWords), 0, 0};
+        std::vector<uint4> stream;  // {x, y, bits(q), tag}
         for (size_t k : of_wave[w]) {
           const size_t i = slots[k].first, j = slots[k].second;
           const int t = cr[i].t;
@@ -910,24 +922,42 @@ void Layer::build_family(int32_t id) {
             u.y = F.unary ? 0u : (uint32_t)((rows_a + cr[m].b - b_lo) * stride * 4);
             std::memcpy(&u.z, &cr[m].q, 4);
             u.w = 0;
-            packed.push_back(u);
+            stream.push_back(u);
           }
           for (size_t m = j - i; m < padded(j - i); ++m) {
             uint4 u{0u, 0u, 0u, 0u};
             std::memcpy(&u.z, &null_q, 4);
-            packed.push_back(u);
+            stream.push_back(u);
           }
-          packed.back().w = kSlotEnd | (uint32_t)bot[t];
+          stream.back().w = kSlotEnd | (uint32_t)bot[t];
         }
-        ww.z = (int32_t)packed.size();
+        while (stream.size() % kGroupRules) {
+          uint4 u{0u, 0u, 0u, 0u};
+          std::memcpy(&u.z, &null_q, 4);
+          stream.push_back(u);
+        }
+        for (size_t g0 = 0; g0 < stream.size(); g0 += kGroupRules) {
+          uint32_t words[kGroupWords] = {};
+          for (int u = 0; u < kGroupRules; ++u) {
+            const uint4& r = stream[g0 + u];
+            if (r.x > kOffsetMask || r.y > kOffsetMask) throw std::logic_error("LDS offset out of range");
+            if (r.w != 0 && u % kRuleUnroll != kRuleUnroll - 1) throw std::logic_error("slot end inside a half group");
+            const uint32_t tag = r.w ? (kSlotEnd | (r.w & 127u) << 24) : 0u;
+            words[u] = r.x | tag;
+            words[kGroupRules + u] = r.y;
+            words[2 * kGroupRules + u] = r.z;
+          }
+          packed.insert(packed.end(), words, words + kGroupWords);
+        }
+        ww.z = (int32_t)(packed.size() / kGroupWords);
         wave_work.push_back(ww);
       }
       F.h_chunk_slot_start.push_back((int32_t)F.h_slot_target.size());
       ++F.num_chunks;
     }
-  // The rule loop reads two groups past the end of a wave's range: a tail of null records keeps
+  // The rule loop reads two groups past the end of a wave's range: a tail of null groups keeps
   // those reads inside the array (their contents are never used).
-  for (int k = 0; k < 2 * kRuleUnroll; ++k) packed.push_back(uint4{0u, 0u, 0u, 0u});
+  packed.resize(packed.size() + 2 * kGroupWords, 0u);
   F.num_slots = (int32_t)F.h_slot_target.size();
   F.d_chunk_range = upload(F.h_chunk_range);
   F.d_slot_target = upload(F.h_slot_target);

@@ -10,8 +10,15 @@ namespace puck {
 constexpr int kMaxGroupFamilies = 8;
 constexpr int kBlockThreads = 1024;   // rule kernels: 16 waves share one LDS item tile
 constexpr int kLdsBudgetBytes = 150 * 1024;
-constexpr int kRuleUnroll = 4;        // rules per pipelined group; slot rule lists are padded to it
-constexpr uint32_t kSlotEnd = 0x80000000u;  // flag in the last rule record of a slot
+constexpr int kRuleUnroll = 4;        // slot rule lists are padded to a multiple of this
+constexpr int kGroupRules = 8;        // rules per pipelined group of the rule stream
+// A group is stored as kGroupWords dwords: A-row words [0, 8), B-row byte offsets [8, 16), weight
+// bits [16, 24). An A-row word holds the byte offset in bits [0, 18); the words of rules 3 and 7
+// (the last rule of each half group) carry the half group's tag in bits [24, 32): kSlotEnd when
+// a slot ends there, with the slot's block bit in bits [24, 31).
+constexpr int kGroupWords = 24;
+constexpr uint32_t kOffsetMask = (1u << 18) - 1;
+constexpr uint32_t kSlotEnd = 0x80000000u;
 constexpr int kReduceTargets = 16;    // reduce kernel: targets (waves) per workgroup
 
 // Arguments of the rule-family kernel (one launch = one family, one span, one cell slice).
@@ -27,8 +34,8 @@ struct FamArgs {
   const int32_t* cell_list;  // slice cells, packed (sentence << 8) | begin
   const int32_t* prefix;     // slice item prefix (num_cells + 1), null = uniform count
   const int32_t* chunk_range;
-  const int4* wave_work;  // per (chunk, wave): {first slot, first rule, end rule, 0}
-  const uint4* rules;  // {A row byte offset, B row byte offset, bits(q), kSlotEnd | block or 0}
+  const int4* wave_work;  // per (chunk, wave): {first slot, first group, end group, 0}
+  const uint32_t* rules;  // rule groups, kGroupWords dwords each
   const int32_t* block_of_target;
   const uint32_t* block_masks;
   float* partial;
