@@ -95,8 +95,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     if (item < a.num_items) {
       int k, j;
       if (a.prefix == nullptr) {
-        k = item / a.uniform_count;
-        j = item - k * a.uniform_count;
+        // Uniform families are split-major: item = split * num_cells + cell, so that the reduce
+        // reads each split's partials of consecutive cells as one coalesced row.
+        j = item / a.num_cells;
+        k = item - j * a.num_cells;
       } else {
         const int abs_item = item + a.item_base;
         int lo = 0, hi = a.num_cells;  // largest k with prefix[k] <= abs_item
@@ -333,14 +335,34 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
     for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
       const int f = a.fam[e];
       const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
-      int lo0, hi0, my_lo, my_hi;
       if (a.prefix[f] == nullptr) {
+        // Split-major items: split j of cell k is item j * num_cells + k; lanes read consecutive
+        // cells of one split row.
         const int uc = a.uniform_count[f];
-        lo0 = k0 * uc;
-        hi0 = kend * uc;
-        my_lo = my_k * uc;
-        my_hi = my_k < kend ? my_lo + uc : my_lo;
-      } else {
+        if (my_k < kend) {
+          const float* q = p + my_k;
+          int j = 0;
+          for (; j + 4 <= uc; j += 4) {
+            const float v0 = q[(size_t)j * a.num_cells], v1 = q[(size_t)(j + 1) * a.num_cells];
+            const float v2 = q[(size_t)(j + 2) * a.num_cells], v3 = q[(size_t)(j + 3) * a.num_cells];
+            if (SEMI == kViterbi) {
+              acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
+            } else {
+              acc += v0;
+              acc += v1;
+              acc += v2;
+              acc += v3;
+            }
+          }
+          for (; j < uc; ++j) {
+            const float v = q[(size_t)j * a.num_cells];
+            if (SEMI == kViterbi) acc = fmaxf(acc, v); else acc += v;
+          }
+        }
+        continue;
+      }
+      int lo0, hi0, my_lo, my_hi;
+      {
         const int32_t* pre = a.prefix[f];
         const int base = a.base[f];
         lo0 = pre[k0] - base;
