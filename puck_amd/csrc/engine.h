@@ -63,7 +63,12 @@ struct DevFamily {
   std::vector<int32_t> h_slot_target;
   int32_t* d_chunk_range = nullptr;
   int32_t* d_slot_target = nullptr;
-  int4* d_wave_work = nullptr;  // per (chunk, wave): {first slot, first group, end group, 0}
+  std::vector<int32_t> h_slot_block;
+  int32_t num_block_ranges = 1;  // block ranges per wave stream (blocks in block-major layouts)
+  int32_t sentinel_group = 0;    // index of the null group after the last wave's range
+  bool skip_layout = false;      // block-major stream of kSkipGroupWords groups
+  int32_t* d_wave_blocks = nullptr;  // per (chunk, wave): num_block_ranges + 1 group starts, first slot
+  int32_t* d_slot_block = nullptr;   // block bit of each slot
   uint32_t* d_rules = nullptr;  // rule groups (kGroupWords dwords each)
   // Block gating (CLParser.scala:783-785,845-855): block of each target symbol and block masks.
   int32_t num_blocks = 0;
@@ -85,13 +90,16 @@ enum ReduceId : int32_t { kRInBin = 0, kRInBin2, kRInUN, kRInUT, kROutBin, kROut
 
 class Layer {
  public:
-  Layer(Grammar g, bool final_layer, int32_t items_per_lane);
+  // masked: the layer runs under the pruning masks of the layer before it (every layer but the
+  // first); its binary rule streams are then laid out block-major.
+  Layer(Grammar g, bool final_layer, bool masked, int32_t items_per_lane);
   ~Layer();
   Layer(const Layer&) = delete;
   Layer& operator=(const Layer&) = delete;
 
   Grammar g;
   bool final_layer;
+  bool masked;
   int32_t ipl;  // items per lane of the rule kernels (1 or 2)
   DevFamily fam[kNumFamilies];
   DevReduce red[kNumReduce];
@@ -188,7 +196,7 @@ class Parser {
   // Workspace (grown on demand).
   struct Buf { void* p = nullptr; size_t bytes = 0; };
   Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
-      pos_scale_, aux_;
+      pos_scale_, aux_, mplan_;
   void ensure(Buf& b, size_t bytes);
 
   void run_batch(const int32_t* words, const int32_t* offsets, int32_t first, int32_t count,
@@ -199,7 +207,8 @@ class Parser {
   void inside_pass(Batch& b, int32_t li, bool use_masks);
   void outside_pass(Batch& b, int32_t li, bool use_masks);
   void launch_family(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks,
-                     int32_t slice_cell0, int32_t slice_cells, float* partial_base);
+                     int32_t slice_cell0, int32_t slice_cells, float* partial_base,
+                     const int32_t* plan_counts, const int32_t* plan_items);
   void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks);
   void timed_begin();
   void timed_end(double* acc);

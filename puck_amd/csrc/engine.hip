@@ -50,6 +50,35 @@ __device__ inline int cell_abs(const int32_t* cell_offsets, int sent, int n, int
   return cell_offsets[sent] + ((half & 1) ? tri(n) : 0) + chart_index(b, e, n);
 }
 
+// Operand cells of item j of cell (b, e) for a family kind: A = (ax0, ay0), B = (bx0, by0).
+// Split ranges: CLParser.scala:345-350,492-518 (CLWorkQueueKernels.scala:138-168).
+__device__ inline void item_operands(int kind, int b, int e, int j, int& ax0, int& ay0, int& bx0, int& by0) {
+  ax0 = b; ay0 = e; bx0 = 0; by0 = 0;
+  switch (kind) {
+    case 0: { int s = b + 1 + j; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;      // IN range
+    case 1: { int s = e - 1; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;          // IN e-1
+    case 2: { int s = b + 1; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;          // IN b+1
+    case 3: { int s = e + 1 + j; ax0 = b; ay0 = s; bx0 = e; by0 = s; } break;      // OUT_L range
+    case 4: { int s = e + 1; ax0 = b; ay0 = s; bx0 = e; by0 = s; } break;          // OUT_L e+1
+    case 5: { int s = j; ax0 = s; ay0 = b; bx0 = s; by0 = e; } break;              // OUT_R range
+    case 6: { int s = b - 1; ax0 = s; ay0 = b; bx0 = s; by0 = e; } break;          // OUT_R b-1
+    default: break;                                                                 // unary
+  }
+}
+
+// Items a family generates at cell (b, e) of a sentence of length n.
+__host__ __device__ inline int items_for(int kind, int b, int e, int n) {
+  switch (kind) {
+    case 0: return e - b - 1;
+    case 1: case 2: return 1;
+    case 3: return n - e;
+    case 4: return e + 1 <= n ? 1 : 0;
+    case 5: return b;
+    case 6: return b >= 1 ? 1 : 0;
+    default: return 1;
+  }
+}
+
 __device__ inline bool mask_any(const uint32_t* m, int words) {
   uint32_t x = 0;
   for (int i = 0; i < words; ++i) x |= m[i];
@@ -69,17 +98,30 @@ __device__ inline float semi_zero() {
 // with buffer loads. Each slot writes one partial value per item; the reduce kernel combines
 // partials over splits, chunks and families. Rule semantics per SimpleGenRuleMultiply.java:151
 // (parent = semiring_mad(parent, left (x) right, q)) and :213-216 (Real: *= scale).
-template <int IPL, int SEMI, bool UNARY>
+template <int IPL, int SEMI, bool UNARY, bool SKIP>
 __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   constexpr int ITEMS = 64 * IPL;
   extern __shared__ __align__(16) float smem[];
   __shared__ int32_t s_cell_a[ITEMS], s_cell_b[ITEMS];
   __shared__ uint32_t s_gate[4][ITEMS];  // 128 block bits per item
   __shared__ float s_scale[ITEMS];
+  __shared__ uint32_t s_union[4];  // blocks active in any item of the tile
   const float zero = semi_zero<SEMI>();
   const int tid = threadIdx.x;
   const int tile0 = blockIdx.x * ITEMS;
   const int chunk = blockIdx.y;
+  // Items of the slice: [item_begin, item_begin + num_items). Prefix-counted and planned slices
+  // read their bounds on the device (the grid covers the largest possible count).
+  int item_begin = 0, num_items = a.num_items;
+  if (a.prefix != nullptr) {
+    item_begin = a.prefix[0];
+    num_items = a.prefix[a.num_cells] - item_begin;
+  }
+  if (tile0 >= num_items) return;
+  if constexpr (SKIP) {
+    if (tid < 4) s_union[tid] = 0u;
+    __syncthreads();
+  }
   const int a_lo = a.chunk_range[4 * chunk], a_hi = a.chunk_range[4 * chunk + 1];
   const int b_lo = a.chunk_range[4 * chunk + 2], b_hi = a.chunk_range[4 * chunk + 3];
   const int rows_a = a_hi - a_lo;
@@ -92,15 +134,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     int ca = 0, cb = 0;
     uint32_t gate[4] = {0u, 0u, 0u, 0u};
     float scale = 1.0f;
-    if (item < a.num_items) {
+    if (item < num_items) {
       int k, j;
-      if (a.prefix == nullptr) {
+      if (a.plan_items != nullptr) {
+        const int code = a.plan_items[item_begin + item];
+        k = (code >> 8) - a.slice_cell0;
+        j = code & 255;
+      } else if (a.prefix == nullptr) {
         // Uniform families are split-major: item = split * num_cells + cell, so that the reduce
         // reads each split's partials of consecutive cells as one coalesced row.
         j = item / a.num_cells;
         k = item - j * a.num_cells;
       } else {
-        const int abs_item = item + a.item_base;
+        const int abs_item = item + item_begin;
         int lo = 0, hi = a.num_cells;  // largest k with prefix[k] <= abs_item
         while (hi - lo > 1) {
           int mid = (lo + hi) >> 1;
@@ -112,18 +158,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
       const int packed = a.cell_list[k];
       const int sent = packed >> 8, b = packed & 255, e = b + a.span;
       const int n = a.lengths[sent];
-      int ax0 = b, ay0 = e, bx0 = 0, by0 = 0;
-      // Split ranges: CLParser.scala:345-350,492-518 (CLWorkQueueKernels.scala:138-168).
-      switch (a.kind) {
-        case 0: { int s = b + 1 + j; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;      // IN range
-        case 1: { int s = e - 1; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;          // IN e-1
-        case 2: { int s = b + 1; ax0 = b; ay0 = s; bx0 = s; by0 = e; } break;          // IN b+1
-        case 3: { int s = e + 1 + j; ax0 = b; ay0 = s; bx0 = e; by0 = s; } break;      // OUT_L range
-        case 4: { int s = e + 1; ax0 = b; ay0 = s; bx0 = e; by0 = s; } break;          // OUT_L e+1
-        case 5: { int s = j; ax0 = s; ay0 = b; bx0 = s; by0 = e; } break;              // OUT_R range
-        case 6: { int s = b - 1; ax0 = s; ay0 = b; bx0 = s; by0 = e; } break;          // OUT_R b-1
-        default: break;                                                                 // unary
-      }
+      int ax0, ay0, bx0, by0;
+      item_operands(a.kind, b, e, j, ax0, ay0, bx0, by0);
       ca = cell_abs(a.cell_offsets, sent, n, a.a_half, ax0, ay0);
       cb = UNARY ? 0 : cell_abs(a.cell_offsets, sent, n, a.b_half, bx0, by0);
       const int ct = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
@@ -154,7 +190,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     }
     s_cell_a[tid] = ca;
     s_cell_b[tid] = cb;
-    for (int w = 0; w < 4; ++w) s_gate[w][tid] = gate[w];
+    for (int w = 0; w < 4; ++w) {
+      s_gate[w][tid] = gate[w];
+      if (SKIP && gate[w]) atomicOr(&s_union[w], gate[w]);
+    }
     s_scale[tid] = scale;
   }
   __syncthreads();
@@ -172,7 +211,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
         if (idx < total) {
           const int it = idx / rows;
           const int row = idx - it * rows;
-          if (tile0 + it < a.num_items) v[k] = ch[(size_t)cells[it] * a.cell_size + lo + row];
+          if (tile0 + it < num_items) v[k] = ch[(size_t)cells[it] * a.cell_size + lo + row];
         }
       }
 #pragma unroll
@@ -200,19 +239,60 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   // group of eight rules for the whole wave).
   typedef uint32_t W16 __attribute__((ext_vector_type(16)));
   typedef uint32_t W8 __attribute__((ext_vector_type(8)));
-  typedef uint32_t W4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t W2 __attribute__((ext_vector_type(2)));
   typedef const __attribute__((address_space(4))) W16 ConstW16;
   typedef const __attribute__((address_space(4))) W8 ConstW8;
-  typedef const __attribute__((address_space(4))) W4 ConstW4;
+  typedef const __attribute__((address_space(4))) W2 ConstW2;
   typedef const __attribute__((address_space(4))) char ConstByte;
+  typedef const __attribute__((address_space(4))) int32_t ConstI32;
+  // Groups are addressed by 32-bit byte offsets from the start of the family's stream.
+  constexpr uint32_t GB = (SKIP ? kSkipGroupWords : kGroupWords) * 4;
   ConstByte* gbase = (ConstByte*)a.rules;
-  auto group_off = [&](int g) { return *(ConstW16*)(gbase + (size_t)g * (kGroupWords * 4)); };
-  auto group_q = [&](int g) { return *(ConstW8*)(gbase + (size_t)g * (kGroupWords * 4) + 64); };
-  const W4 work = ((ConstW4*)a.wave_work)[chunk * kWaves + wave];
-  int slot = (int)work.x;
-  int gi = (int)work.y;
-  const int g_end = (int)work.z;
-  if (gi >= g_end) return;
+  auto group_off = [&](uint32_t o) { return *(ConstW16*)(gbase + o); };
+  auto group_q = [&](uint32_t o) { return *(ConstW8*)(gbase + o + 64); };
+  auto group_slots = [&](uint32_t o) { return *(ConstW2*)(gbase + o + 96); };
+
+  // Block ranges of this wave's stream, visited in order. A skip layout lets a masked tile pass
+  // over the blocks that none of its items allows; the cursor (cb, c_end) belongs to the furthest
+  // group fetched so far. Otherwise the wave streams one range and counts its slots.
+  const int nbr = a.num_block_ranges;
+  ConstI32* bt = (ConstI32*)a.wave_blocks + (size_t)(chunk * kWaves + wave) * (nbr + 2);
+  uint64_t un_lo = ~0ull, un_hi = ~0ull;  // block bits 0-63 and 64-127 of the tile's union
+  if (SKIP && a.masks != nullptr) {
+    un_lo = (uint64_t)__builtin_amdgcn_readfirstlane(s_union[0]) |
+            (uint64_t)__builtin_amdgcn_readfirstlane(s_union[1]) << 32;
+    un_hi = (uint64_t)__builtin_amdgcn_readfirstlane(s_union[2]) |
+            (uint64_t)__builtin_amdgcn_readfirstlane(s_union[3]) << 32;
+  }
+  int cb = -1;
+  uint32_t c_end = 0;  // byte offset of the end of the cursor's range
+  const uint32_t sentinel = (uint32_t)a.sentinel_group * GB;
+  auto next_range = [&]() -> uint32_t {
+    while (++cb < nbr) {
+      const uint64_t word = (cb & 64) ? un_hi : un_lo;
+      if (!((word >> (cb & 63)) & 1u)) continue;
+      const int b0 = bt[cb], b1 = bt[cb + 1];
+      if (b0 < b1) {
+        c_end = (uint32_t)b1 * GB;
+        return (uint32_t)b0 * GB;
+      }
+    }
+    return sentinel;
+  };
+  auto advance = [&](uint32_t o) { return o + GB < c_end ? o + GB : next_range(); };
+  uint32_t gi, g1, g_end = 0;
+  uint32_t slot_ctr = 0;
+  if constexpr (SKIP) {
+    gi = next_range();
+    if (gi == sentinel) return;
+    g1 = advance(gi);
+  } else {
+    gi = (uint32_t)bt[0] * GB;
+    g_end = (uint32_t)bt[1] * GB;
+    slot_ctr = (uint32_t)bt[2];
+    if (gi >= g_end) return;
+    g1 = gi + GB;
+  }
 
   // Per-item gate words and scale, kept in registers for the slot flushes.
   uint64_t gate_lo[IPL], gate_hi[IPL];  // block bits 0-63 and 64-127
@@ -247,7 +327,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   float acc[IPL];
 #pragma unroll
   for (int i = 0; i < IPL; ++i) acc[i] = zero;
-  auto flush = [&](uint32_t tag) {
+  auto flush = [&](uint32_t tag, uint32_t slot) {
     // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
     const int bit = (int)((tag >> 24) & 127u);
     float* out = a.partial + (size_t)slot * a.pitch;
@@ -257,12 +337,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
       float v = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
       if (SEMI == kReal) v *= scl[i];
       const int item = tile0 + lane * IPL + i;
-      if (item < a.num_items) out[item] = v;
+      if (item < num_items) out[item] = v;
       acc[i] = zero;
     }
-    ++slot;
   };
-  auto half = [&](const W8& qs, uint32_t tag, float (*da)[IPL], float (*db)[IPL], int h) {
+  auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, float (*da)[IPL], float (*db)[IPL], int h) {
 #pragma unroll
     for (int u = h * H; u < h * H + H; ++u) {
       const float q = __uint_as_float(qs[u]);
@@ -276,40 +355,123 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
         }
       }
     }
-    if (tag & kSlotEnd) flush(tag);
+    if (tag & kSlotEnd) {
+      if constexpr (SKIP) {
+        flush(tag, slot);
+      } else {
+        flush(tag, slot_ctr);
+        ++slot_ctr;
+      }
+    }
   };
-  // Software pipeline over groups: while group g is computed from (da, db) with weights cur_q
-  // and tags (cur_t0, cur_t1), the LDS reads of group g+1 (offsets nxt_off) go to (la, lb), the
-  // offsets of group g+2 and the weights of group g+1 are loaded into SGPRs. Scalar loads share
-  // the LDS wait counter, so the one wait the compiler places before the next step's LDS reads
-  // covers both. Register sets alternate between two unrolled steps, so nothing is copied.
-  // Groups past g_end belong to the next wave or to the null tail: read, never used.
-  auto step = [&](const W8& cur_q, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
-                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q,
+  // Software pipeline over groups: while group gi is computed from (da, db) with weights cur_q
+  // and tags (cur_t0, cur_t1), the LDS reads of group g1 (offsets nxt_off) go to (la, lb), and the
+  // offsets of the group after g1 and the weights (and, in skip layouts, slot ids) of g1 are
+  // loaded into SGPRs. Scalar loads share the LDS wait counter, so the one wait the compiler
+  // places before the next step's LDS reads covers both. Register sets alternate between two
+  // unrolled steps, so nothing is copied.
+  auto step = [&](const W8& cur_q, const W2& cur_s, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
+                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q, W2& nxt_s,
                   float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
     read_group(nxt_off, la, lb);
     nxt_t0 = nxt_off[H - 1];
     nxt_t1 = nxt_off[G - 1];
-    nn_off = group_off(gi + 2);
-    nxt_q = group_q(gi + 1);
+    const uint32_t g2 = SKIP ? advance(g1) : g1 + GB;
+    nn_off = group_off(g2);
+    nxt_q = group_q(g1);
+    if constexpr (SKIP) nxt_s = group_slots(g1);
     __builtin_amdgcn_sched_barrier(0);
-    half(cur_q, cur_t0, da, db, 0);
-    half(cur_q, cur_t1, da, db, 1);
+    half(cur_q, cur_t0, cur_s.x, da, db, 0);
+    half(cur_q, cur_t1, cur_s.y, da, db, 1);
     __builtin_amdgcn_sched_barrier(0);
-    ++gi;
+    gi = g1;
+    g1 = g2;
   };
-  W16 oA = group_off(gi), oB = group_off(gi + 1);
+  // Skip layouts end at the null sentinel group; plain layouts at the wave's range end (the
+  // groups after it, read but never used, belong to the next wave or to the null tail).
+  auto done = [&]() { return SKIP ? gi == sentinel : gi >= g_end; };
+  W16 oA = group_off(gi), oB = group_off(g1);
   W8 qA = group_q(gi), qB;
+  W2 sA = {0u, 0u}, sB = {0u, 0u};
+  if constexpr (SKIP) sA = group_slots(gi);
   uint32_t tA0 = oA[H - 1], tA1 = oA[G - 1], tB0 = 0, tB1 = 0;
   float a0[G][IPL], b0[G][IPL], a1[G][IPL], b1[G][IPL];
   read_group(oA, a0, b0);
   for (;;) {
-    // cur = A (data 0); reads group g+1 from oB into data 1; loads offsets g+2 into oA.
-    step(qA, tA0, tA1, oB, tB0, tB1, oA, qB, a0, b0, a1, b1);
-    if (gi >= g_end) break;
-    step(qB, tB0, tB1, oA, tA0, tA1, oB, qA, a1, b1, a0, b0);
-    if (gi >= g_end) break;
+    step(qA, sA, tA0, tA1, oB, tB0, tB1, oA, qB, sB, a0, b0, a1, b1);
+    if (done()) break;
+    step(qB, sB, tB0, tB1, oA, tA0, tA1, oB, qA, sA, a1, b1, a0, b0);
+    if (done()) break;
   }
+}
+
+// Mask plan of one span: the items each family must compute under the pruning masks, as the
+// reference's update managers enumerate them (CLParser.scala:838-925): a binary item needs a
+// target cell whose mask meets some block of the family (:845-855) and allowed child spans
+// (:892-893); a unary item needs a non-empty top mask (:747). Counts, then an exclusive scan per
+// family, then the item lists ((cell << 8) | split, cell-major).
+template <bool FILL>
+__global__ void k_plan(PlanArgs p) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const PlanFam& f = p.fam[blockIdx.y];
+  if (k >= p.num_cells) return;
+  const int packed = p.cell_list[k];
+  const int sent = packed >> 8, b = packed & 255, e = b + p.span;
+  const int n = p.lengths[sent];
+  int count = 0;
+  int32_t* out = FILL ? f.items + f.counts[k] : nullptr;
+  if (f.unary) {
+    if (mask_any(p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, 1, b, e) * p.mask_words, p.mask_words)) {
+      if (FILL) out[0] = k << 8;
+      count = 1;
+    }
+  } else {
+    const uint32_t* tm = p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, f.t_half, b, e) * p.mask_words;
+    bool active = false;
+    for (int bb = 0; bb < f.num_blocks && !active; ++bb)
+      for (int w = 0; w < p.mask_words; ++w) active |= (tm[w] & f.block_masks[bb * p.mask_words + w]) != 0u;
+    if (active) {
+      const int m = items_for(f.kind, b, e, n);
+      for (int j = 0; j < m; ++j) {
+        int ax0, ay0, bx0, by0;
+        item_operands(f.kind, b, e, j, ax0, ay0, bx0, by0);
+        const int ca = cell_abs(p.cell_offsets, sent, n, f.a_half, ax0, ay0);
+        const int cb = cell_abs(p.cell_offsets, sent, n, f.b_half, bx0, by0);
+        if (mask_any(p.masks + (size_t)ca * p.mask_words, p.mask_words) &&
+            mask_any(p.masks + (size_t)cb * p.mask_words, p.mask_words)) {
+          if (FILL) out[count] = (k << 8) | j;
+          ++count;
+        }
+      }
+    }
+  }
+  if (!FILL) f.counts[k] = count;
+}
+
+// In-place exclusive scan of counts[0, num_cells) into counts[0, num_cells]; one block per family.
+__global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
+  __shared__ int32_t s_sum[1024];
+  int32_t* c = p.fam[blockIdx.y].counts;
+  const int t = threadIdx.x;
+  const int per = (p.num_cells + 1023) / 1024;
+  const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
+  int32_t sum = 0;
+  for (int i = lo; i < hi; ++i) sum += c[i];
+  s_sum[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int32_t v = t >= d ? s_sum[t - d] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int32_t run = s_sum[t] - sum;  // exclusive prefix of this thread's range
+  for (int i = lo; i < hi; ++i) {
+    const int32_t v = c[i];
+    c[i] = run;
+    run += v;
+  }
+  if (t == 1023) c[p.num_cells] = s_sum[1023];
 }
 
 // Combine partials: target[cell][t] = (+) over (family, slot) of t over the cell's items.
@@ -331,15 +493,30 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   const int my_k = k0 + lane;
   float acc = semi_zero<SEMI>();
   float* buf = s_buf[w];
+  // Masked layers: this lane's target-cell mask, for the active-block test of each slot.
+  const uint32_t* tmask = nullptr;
+  if (a.masks != nullptr && my_k < kend) {
+    const int packed = a.cell_list[my_k];
+    const int sent = packed >> 8, b = packed & 255;
+    const int n = a.lengths[sent];
+    tmask = a.masks + (size_t)cell_abs(a.cell_offsets, sent, n, a.t_half, b, b + a.span) * a.mask_words;
+  }
   if (t < a.n_t) {
     for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
       const int f = a.fam[e];
       const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
+      bool use = true;
+      if (tmask != nullptr && a.slot_block[f] != nullptr) {
+        const uint32_t* bm = a.block_masks[f] + (size_t)a.slot_block[f][a.slot[e]] * a.mask_words;
+        uint32_t hit = 0;
+        for (int m = 0; m < a.mask_words; ++m) hit |= tmask[m] & bm[m];
+        use = hit != 0u;
+      }
       if (a.prefix[f] == nullptr) {
         // Split-major items: split j of cell k is item j * num_cells + k; lanes read consecutive
         // cells of one split row.
         const int uc = a.uniform_count[f];
-        if (my_k < kend) {
+        if (my_k < kend && use) {
           const float* q = p + my_k;
           int j = 0;
           for (; j + 4 <= uc; j += 4) {
@@ -364,7 +541,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
       int lo0, hi0, my_lo, my_hi;
       {
         const int32_t* pre = a.prefix[f];
-        const int base = a.base[f];
+        const int base = pre[0];
         lo0 = pre[k0] - base;
         hi0 = pre[kend] - base;
         my_lo = my_k < kend ? pre[my_k] - base : hi0;
@@ -382,7 +559,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int jz = min(my_hi, pb + len);
+        const int jz = use ? min(my_hi, pb + len) : 0;
         for (int j = max(my_lo, pb); j < jz; ++j) {
           const float v = buf[j - pb];
           if (SEMI == kViterbi) acc = fmaxf(acc, v); else acc += v;
@@ -734,8 +911,8 @@ T* Layer::upload(const std::vector<T>& v) {
   return p;
 }
 
-Layer::Layer(Grammar gr, bool final_l, int32_t items_per_lane)
-    : g(std::move(gr)), final_layer(final_l), ipl(items_per_lane) {
+Layer::Layer(Grammar gr, bool final_l, bool masked_l, int32_t items_per_lane)
+    : g(std::move(gr)), final_layer(final_l), masked(masked_l), ipl(items_per_lane) {
   if (ipl != 1 && ipl != 2) throw std::runtime_error("items per lane must be 1 or 2");
   if (g.mask_words > 4) throw std::runtime_error("at most 128 coarse symbols are supported");
   for (int f = 0; f < kNumFamilies; ++f) build_family(f);
@@ -885,15 +1062,26 @@ void Layer::build_family(int32_t id) {
   }
 
   // Rule stream. Per chunk, the target symbols (slots) are assigned to the workgroup's waves by
-  // longest-processing-time-first over their padded rule counts; each wave's slots are laid out
-  // back to back so that the wave streams one contiguous rule range. Slot ids follow that layout.
-  // A slot's rules are padded with null rules (semiring-zero weight) to a multiple of kRuleUnroll
-  // and the last rule of the slot is tagged kSlotEnd | block bit. Each wave's stream is padded to
-  // whole groups of kGroupRules and stored group-major (see kGroupWords).
+  // longest-processing-time-first over their padded rule counts, and each wave streams its slots
+  // back to back. A slot's rules are padded with null rules (semiring-zero weight) to a multiple of
+  // kRuleUnroll; its last rule carries the slot-end tag and the slot id travels with the group
+  // (see kGroupWords). In masked layers with several blocks the stream of each wave is block-major
+  // (blocks padded to whole groups) and a per-wave table of block ranges lets a tile skip the
+  // blocks that none of its cells allows.
   constexpr int kWaves = kBlockThreads / 64;
+  const bool block_major = masked && !F.unary && F.num_blocks > 1;
+  F.skip_layout = block_major;
+  F.num_block_ranges = block_major ? F.num_blocks : 1;
+  const int gw = block_major ? kSkipGroupWords : kGroupWords;
   std::vector<uint32_t> packed;  // kGroupWords per group
-  std::vector<int4> wave_work;
+  std::vector<int32_t> wave_blocks;
   const float null_q = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  struct SRule { uint32_t x, y, q, tag; int32_t slot; };
+  auto null_rule = [&]() {
+    SRule u{0u, 0u, 0u, 0u, 0};
+    std::memcpy(&u.q, &null_q, 4);
+    return u;
+  };
   F.h_chunk_slot_start.push_back(0);
   for (int ia = 0; ia < ca; ++ia)
     for (int ib = 0; ib < (F.unary ? 1 : cb); ++ib) {
@@ -918,73 +1106,81 @@ void Layer::build_family(int32_t id) {
         i = j;
       }
       auto padded = [](size_t n) { return (n + kRuleUnroll - 1) / kRuleUnroll * kRuleUnroll; };
-      std::vector<size_t> order(slots.size());
-      for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-      std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-        return slots[x].second - slots[x].first > slots[y].second - slots[y].first;
-      });
-      // Per slot the flush costs about two rule groups; count it in the balance.
+      const int nbr = F.num_block_ranges;
+      // of_wave[w][b]: slots of block range b streamed by wave w.
+      std::vector<std::vector<std::vector<size_t>>> of_wave(kWaves, std::vector<std::vector<size_t>>(nbr));
       std::vector<int64_t> load(kWaves, 0);
-      std::vector<std::vector<size_t>> of_wave(kWaves);
-      for (size_t k : order) {
-        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[w] += (int64_t)padded(slots[k].second - slots[k].first) + 2 * kRuleUnroll;
-        of_wave[w].push_back(k);
+      for (int br = 0; br < nbr; ++br) {
+        std::vector<size_t> order;
+        for (size_t k = 0; k < slots.size(); ++k)
+          if (!block_major || bot[cr[slots[k].first].t] == br) order.push_back(k);
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+          return slots[x].second - slots[x].first > slots[y].second - slots[y].first;
+        });
+        // Per slot the flush costs about two rule groups; count it in the balance.
+        for (size_t k : order) {
+          const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+          load[w] += (int64_t)padded(slots[k].second - slots[k].first) + 2 * kRuleUnroll;
+          of_wave[w][br].push_back(k);
+        }
       }
       for (int w = 0; w < kWaves; ++w) {
-        int4 ww{(int32_t)F.h_slot_target.size(), (int32_t)(packed.size() / kGroupWords), 0, 0};
-        std::vector<uint4> stream;  // {x, y, bits(q), tag}
-        for (size_t k : of_wave[w]) {
-          const size_t i = slots[k].first, j = slots[k].second;
-          const int t = cr[i].t;
-          F.h_slot_target.push_back(t);
-          for (size_t m = i; m < j; ++m) {
-            uint4 u;
-            u.x = (uint32_t)((cr[m].a - a_lo) * stride * 4);
-            u.y = F.unary ? 0u : (uint32_t)((rows_a + cr[m].b - b_lo) * stride * 4);
-            std::memcpy(&u.z, &cr[m].q, 4);
-            u.w = 0;
-            stream.push_back(u);
+        std::vector<SRule> stream;
+        const int32_t first_slot = (int32_t)F.h_slot_target.size();
+        for (int br = 0; br < nbr; ++br) {
+          wave_blocks.push_back((int32_t)(packed.size() / gw + stream.size() / kGroupRules));
+          for (size_t k : of_wave[w][br]) {
+            const size_t i = slots[k].first, j = slots[k].second;
+            const int t = cr[i].t;
+            const int32_t slot_id = (int32_t)F.h_slot_target.size();
+            F.h_slot_target.push_back(t);
+            F.h_slot_block.push_back(bot[t]);
+            for (size_t m = i; m < j; ++m) {
+              SRule u{(uint32_t)((cr[m].a - a_lo) * stride * 4),
+                      F.unary ? 0u : (uint32_t)((rows_a + cr[m].b - b_lo) * stride * 4), 0u, 0u, 0};
+              std::memcpy(&u.q, &cr[m].q, 4);
+              stream.push_back(u);
+            }
+            for (size_t m = j - i; m < padded(j - i); ++m) stream.push_back(null_rule());
+            stream.back().tag = kSlotEnd | ((uint32_t)bot[t] & 127u) << 24;
+            stream.back().slot = slot_id;
           }
-          for (size_t m = j - i; m < padded(j - i); ++m) {
-            uint4 u{0u, 0u, 0u, 0u};
-            std::memcpy(&u.z, &null_q, 4);
-            stream.push_back(u);
-          }
-          stream.back().w = kSlotEnd | (uint32_t)bot[t];
+          if (block_major || br == nbr - 1)
+            while (stream.size() % kGroupRules) stream.push_back(null_rule());
         }
-        while (stream.size() % kGroupRules) {
-          uint4 u{0u, 0u, 0u, 0u};
-          std::memcpy(&u.z, &null_q, 4);
-          stream.push_back(u);
-        }
+        wave_blocks.push_back((int32_t)(packed.size() / gw + stream.size() / kGroupRules));
+        wave_blocks.push_back(first_slot);
         for (size_t g0 = 0; g0 < stream.size(); g0 += kGroupRules) {
-          uint32_t words[kGroupWords] = {};
+          uint32_t words[kSkipGroupWords] = {};
           for (int u = 0; u < kGroupRules; ++u) {
-            const uint4& r = stream[g0 + u];
+            const SRule& r = stream[g0 + u];
             if (r.x > kOffsetMask || r.y > kOffsetMask) throw std::logic_error("LDS offset out of range");
-            if (r.w != 0 && u % kRuleUnroll != kRuleUnroll - 1) throw std::logic_error("slot end inside a half group");
-            const uint32_t tag = r.w ? (kSlotEnd | (r.w & 127u) << 24) : 0u;
-            words[u] = r.x | tag;
+            if (r.tag != 0 && u % kRuleUnroll != kRuleUnroll - 1)
+              throw std::logic_error("slot end inside a half group");
+            words[u] = r.x | r.tag;
             words[kGroupRules + u] = r.y;
-            words[2 * kGroupRules + u] = r.z;
+            words[2 * kGroupRules + u] = r.q;
+            if (r.tag && block_major) words[3 * kGroupRules + u / kRuleUnroll] = (uint32_t)r.slot;
           }
-          packed.insert(packed.end(), words, words + kGroupWords);
+          packed.insert(packed.end(), words, words + gw);
         }
-        ww.z = (int32_t)(packed.size() / kGroupWords);
-        wave_work.push_back(ww);
       }
       F.h_chunk_slot_start.push_back((int32_t)F.h_slot_target.size());
       ++F.num_chunks;
     }
-  // The rule loop reads two groups past the end of a wave's range: a tail of null groups keeps
-  // those reads inside the array (their contents are never used).
-  packed.resize(packed.size() + 2 * kGroupWords, 0u);
+  // The first group past the last wave's range is a null group: the rule loop parks its cursor
+  // there when a wave runs out of work, and the two groups after it keep the prefetch in bounds.
+  F.sentinel_group = (int32_t)(packed.size() / gw);
+  packed.resize(packed.size() + 3 * gw, 0u);
+  for (int k = 0; k < 3; ++k)
+    for (int u = 0; u < kGroupRules; ++u)
+      std::memcpy(&packed[(size_t)(F.sentinel_group + k) * gw + 2 * kGroupRules + u], &null_q, 4);
   F.num_slots = (int32_t)F.h_slot_target.size();
   F.d_chunk_range = upload(F.h_chunk_range);
   F.d_slot_target = upload(F.h_slot_target);
   F.d_rules = upload(packed);
-  F.d_wave_work = upload(wave_work);
+  F.d_wave_blocks = upload(wave_blocks);
+  F.d_slot_block = upload(F.h_slot_block);
 }
 
 void Layer::build_reduce(int32_t rid, std::vector<int32_t> fams, int32_t n_t) {
@@ -1041,26 +1237,25 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
     if (layers[i]->mask_words != layers[0]->mask_words ||
         layers[i]->coarse_names != layers[0]->coarse_names)
       throw std::runtime_error("all layers must share the coarse symbol set");
-    layers_.emplace_back(new Layer(*layers[i], i + 1 == layers.size(), items_per_lane));
+    layers_.emplace_back(new Layer(*layers[i], i + 1 == layers.size(), i > 0, items_per_lane));
   }
   HIP_CHECK(hipStreamCreate(&stream_));
   HIP_CHECK(hipEventCreate(&ev0_));
   HIP_CHECK(hipEventCreate(&ev1_));
   // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
   const int lds = kLdsBudgetBytes;
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kViterbi, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kViterbi, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kReal, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<2, kReal, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kViterbi, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kViterbi, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kReal, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_CHECK(hipFuncSetAttribute((const void*)k_rules<1, kReal, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  for (const void* fn : {(const void*)k_rules<2, kViterbi, false, false>, (const void*)k_rules<2, kViterbi, true, false>,
+                         (const void*)k_rules<2, kViterbi, false, true>, (const void*)k_rules<2, kReal, false, false>,
+                         (const void*)k_rules<2, kReal, true, false>, (const void*)k_rules<2, kReal, false, true>,
+                         (const void*)k_rules<1, kViterbi, false, false>, (const void*)k_rules<1, kViterbi, true, false>,
+                         (const void*)k_rules<1, kViterbi, false, true>, (const void*)k_rules<1, kReal, false, false>,
+                         (const void*)k_rules<1, kReal, true, false>, (const void*)k_rules<1, kReal, false, true>})
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
 }
 
 Parser::~Parser() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
-                 &trees_, &mbr_, &pos_scale_, &aux_})
+                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_})
     if (b->p) (void)hipFree(b->p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -1115,23 +1310,12 @@ void Parser::timed_end(double* acc) {
 }
 
 namespace {
-// Items a family generates at cell (b, e) of a sentence of length n.
-int items_for(int kind, int b, int e, int n) {
-  switch (kind) {
-    case 0: return e - b - 1;
-    case 1: case 2: return 1;
-    case 3: return n - e;
-    case 4: return e + 1 <= n ? 1 : 0;
-    case 5: return b;
-    case 6: return b >= 1 ? 1 : 0;
-    default: return 1;
-  }
-}
 bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
 }  // namespace
 
 void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks,
-                           int32_t k0, int32_t kc, float* partial) {
+                           int32_t k0, int32_t kc, float* partial, const int32_t* plan_counts,
+                           const int32_t* plan_items) {
   Layer& L = *layers_[li];
   const DevFamily& F = L.fam[fid];
   const Grammar& g = L.g;
@@ -1151,22 +1335,34 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   a.cell_list = B.d_cells + B.span_start[span] + k0;
   a.num_cells = kc;
   int64_t nitems;
-  if (uniform_kind(F.kind)) {
+  a.slice_cell0 = k0;
+  a.plan_items = plan_items;
+  if (plan_items != nullptr) {
+    // Planned slice: its item count is on the device; the grid covers the unpruned count.
+    a.prefix = plan_counts + k0;
+    a.uniform_count = 1;
+    if (uniform_kind(F.kind)) {
+      nitems = (int64_t)kc * (F.kind == 0 ? span - 1 : 1);
+    } else {
+      const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+      nitems = B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
+    }
+  } else if (uniform_kind(F.kind)) {
     a.prefix = nullptr;
     a.uniform_count = F.kind == 0 ? span - 1 : 1;
     nitems = (int64_t)kc * a.uniform_count;
-    a.item_base = 0;
   } else {
     const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
     a.prefix = B.d_prefix + po + k0;
-    a.item_base = B.h_prefix[po + k0];
-    nitems = B.h_prefix[po + k0 + kc] - a.item_base;
+    nitems = B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
     a.uniform_count = 1;
   }
   if (nitems <= 0) return;
   a.num_items = (int32_t)nitems;
   a.chunk_range = F.d_chunk_range;
-  a.wave_work = F.d_wave_work;
+  a.wave_blocks = F.d_wave_blocks;
+  a.num_block_ranges = F.num_block_ranges;
+  a.sentinel_group = F.sentinel_group;
   a.rules = F.d_rules;
   a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
   a.block_masks = F.unary ? nullptr : F.d_block_masks;
@@ -1190,11 +1386,13 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   launch_semi(g.semiring, [&](auto semi) {
     constexpr int S = decltype(semi)::value;
     if (L.ipl == 2) {
-      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else hipLaunchKernelGGL((k_rules<2, S, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<2, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<2, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, a);
     } else {
-      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else hipLaunchKernelGGL((k_rules<1, S, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<1, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<1, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, a);
     }
   });
   HIP_CHECK(hipGetLastError());
@@ -1255,6 +1453,60 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
     return B.h_prefix[po + k + 1] - B.h_prefix[po + k];
   };
+  // Under masks, plan the span first: per family, the counts/prefix (ncells + 1) and the item
+  // list (at most the unpruned item count), all on the device.
+  std::vector<const int32_t*> plan_counts(R.families.size(), nullptr), plan_items(R.families.size(), nullptr);
+  const uint32_t* masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words
+                                    : nullptr;
+  if (use_masks) {
+    std::vector<size_t> oc(R.families.size()), oi(R.families.size());
+    size_t total = 0;
+    for (size_t fi = 0; fi < R.families.size(); ++fi) {
+      const int fid = R.families[fi];
+      int64_t maxi = 0;
+      if (uniform_kind(L.fam[fid].kind)) {
+        maxi = (int64_t)ncells * (L.fam[fid].kind == 0 ? span - 1 : 1);
+      } else {
+        const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+        maxi = B.h_prefix[po + ncells] - B.h_prefix[po];
+      }
+      oc[fi] = total;
+      total += (size_t)ncells + 1;
+      oi[fi] = total;
+      total += (size_t)std::max<int64_t>(maxi, 1);
+    }
+    ensure(mplan_, total * 4);
+    PlanArgs pa{};
+    int32_t* mp = (int32_t*)mplan_.p;
+    for (size_t fi = 0; fi < R.families.size(); ++fi) {
+      const DevFamily& F = L.fam[R.families[fi]];
+      PlanFam& pf = pa.fam[fi];
+      pf.kind = F.kind;
+      pf.unary = F.unary;
+      pf.a_half = F.a_half;
+      pf.b_half = F.b_half;
+      pf.t_half = F.t_half;
+      pf.num_blocks = F.unary ? 0 : F.num_blocks;
+      pf.block_masks = F.unary ? nullptr : F.d_block_masks;
+      pf.counts = mp + oc[fi];
+      pf.items = mp + oi[fi];
+      plan_counts[fi] = pf.counts;
+      plan_items[fi] = pf.items;
+    }
+    pa.masks = masks;
+    pa.cell_offsets = B.d_cell_offsets;
+    pa.lengths = B.d_lengths;
+    pa.cell_list = B.d_cells + B.span_start[span];
+    pa.num_cells = ncells;
+    pa.span = span;
+    pa.mask_words = g.mask_words;
+    const unsigned nf = (unsigned)R.families.size();
+    const dim3 pg((ncells + 255) / 256, nf);
+    hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, stream_, pa);
+    hipLaunchKernelGGL(k_plan_scan, dim3(1, nf), dim3(1024), 0, stream_, pa);
+    hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, stream_, pa);
+    HIP_CHECK(hipGetLastError());
+  }
   // Each family's partial region is num_slots x pitch floats, pitch = items rounded up to 256.
   const int64_t budget_floats = (int64_t)(partial_.bytes / 4);
   int64_t fixed = 0;
@@ -1283,19 +1535,23 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       if (uniform_kind(F.kind)) {
         ra.prefix[fi] = nullptr;
         ra.uniform_count[fi] = F.kind == 0 ? span - 1 : 1;
-        ra.base[fi] = 0;
         nitems = (int64_t)kc * ra.uniform_count[fi];
       } else {
         const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
         ra.prefix[fi] = B.d_prefix + po + k0;
-        ra.base[fi] = B.h_prefix[po + k0];
         ra.uniform_count[fi] = 1;
-        nitems = B.h_prefix[po + k0 + kc] - ra.base[fi];
+        nitems = B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
+      }
+      if (use_masks) {
+        ra.prefix[fi] = plan_counts[fi] + k0;
+        ra.uniform_count[fi] = 1;
+        ra.slot_block[fi] = F.unary ? nullptr : F.d_slot_block;
+        ra.block_masks[fi] = F.unary ? nullptr : F.d_block_masks;
       }
       const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
       ra.partial[fi] = base + off;
       ra.pitch[fi] = pitch;
-      launch_family(B, li, fid, span, use_masks, k0, kc, base + off);
+      launch_family(B, li, fid, span, use_masks, k0, kc, base + off, plan_counts[fi], plan_items[fi]);
       off += (size_t)pitch * F.num_slots;
     }
     float* inside = (float*)inside_.p;
@@ -1303,6 +1559,8 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     const DevFamily& F0 = L.fam[R.families[0]];
     ra.chT = F0.t_half >= 2 ? outside : inside;
     ra.t_half = F0.t_half;
+    ra.masks = masks;
+    ra.mask_words = g.mask_words;
     ra.cell_offsets = B.d_cell_offsets;
     ra.lengths = B.d_lengths;
     ra.cell_list = B.d_cells + B.span_start[span] + k0;

@@ -15,8 +15,11 @@ constexpr int kGroupRules = 8;        // rules per pipelined group of the rule s
 // A group is stored as kGroupWords dwords: A-row words [0, 8), B-row byte offsets [8, 16), weight
 // bits [16, 24). An A-row word holds the byte offset in bits [0, 18); the words of rules 3 and 7
 // (the last rule of each half group) carry the half group's tag in bits [24, 32): kSlotEnd when
-// a slot ends there, with the slot's block bit in bits [24, 31).
+// a slot ends there, with the slot's block bit in bits [24, 31). Streams that tiles may skip
+// through (block-major layouts) use kSkipGroupWords: the same words plus, at [24] and [25], the
+// id of the slot ending with each half group, and padding.
 constexpr int kGroupWords = 24;
+constexpr int kSkipGroupWords = 32;
 constexpr uint32_t kOffsetMask = (1u << 18) - 1;
 constexpr uint32_t kSlotEnd = 0x80000000u;
 constexpr int kReduceTargets = 16;    // reduce kernel: targets (waves) per workgroup
@@ -32,14 +35,19 @@ struct FamArgs {
   const int32_t* cell_offsets;
   const int32_t* lengths;
   const int32_t* cell_list;  // slice cells, packed (sentence << 8) | begin
-  const int32_t* prefix;     // slice item prefix (num_cells + 1), null = uniform count
+  // Slice item prefix (num_cells + 1; items of the slice are [prefix[0], prefix[num_cells])), null
+  // = uniform count. With plan_items, the prefix is the mask plan's and items are listed there.
+  const int32_t* prefix;
+  const int32_t* plan_items;  // planned items of the span: (cell << 8) | split, null = all items
   const int32_t* chunk_range;
-  const int4* wave_work;  // per (chunk, wave): {first slot, first group, end group, 0}
+  // Per (chunk, wave): num_block_ranges + 1 group starts, then the wave's first slot id.
+  const int32_t* wave_blocks;
   const uint32_t* rules;  // rule groups, kGroupWords dwords each
   const int32_t* block_of_target;
   const uint32_t* block_masks;
   float* partial;
-  int32_t num_cells, item_base, num_items, uniform_count;
+  int32_t num_cells, num_items, uniform_count, slice_cell0;
+  int32_t num_block_ranges, sentinel_group;
   int32_t span, kind, range_split, cell_size;
   int32_t a_half, b_half, t_half;
   int32_t mask_words, num_blocks, unary_gate;
@@ -56,13 +64,34 @@ struct RedArgs {
   const int32_t* fam;
   const int32_t* slot;
   const float* partial[kMaxGroupFamilies];
-  const int32_t* prefix[kMaxGroupFamilies];
+  const int32_t* prefix[kMaxGroupFamilies];  // slice item prefix, null = uniform count
   int32_t uniform_count[kMaxGroupFamilies];
-  int32_t base[kMaxGroupFamilies];
   int32_t pitch[kMaxGroupFamilies];
+  // Masked layers: a (cell, slot) pair contributes only when the slot's block is active in the
+  // target cell (its mask meets the block mask); inactive pairs were never computed.
+  const uint32_t* masks;  // null = unmasked
+  const int32_t* slot_block[kMaxGroupFamilies];
+  const uint32_t* block_masks[kMaxGroupFamilies];
+  int32_t mask_words;
   int32_t num_cells, cell_size, n_t, t_half, span;
   int32_t root_init, root;
   float one;
+};
+
+// Mask plan of one span for the families of a reduce group.
+struct PlanFam {
+  int32_t kind, unary, a_half, b_half, t_half, num_blocks;
+  const uint32_t* block_masks;
+  int32_t* counts;  // num_cells + 1: counts, then their exclusive prefix
+  int32_t* items;   // planned items, (cell << 8) | split
+};
+struct PlanArgs {
+  PlanFam fam[kMaxGroupFamilies];
+  const uint32_t* masks;
+  const int32_t* cell_offsets;
+  const int32_t* lengths;
+  const int32_t* cell_list;  // cells of the span
+  int32_t num_cells, span, mask_words;
 };
 
 }  // namespace puck
