@@ -420,16 +420,24 @@ __global__ void k_plan(PlanArgs p) {
   const int n = p.lengths[sent];
   int count = 0;
   int32_t* out = FILL ? f.items + f.counts[k] : nullptr;
+  uint32_t act[4] = {0u, 0u, 0u, 0u};
   if (f.unary) {
     if (mask_any(p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, 1, b, e) * p.mask_words, p.mask_words)) {
       if (FILL) out[0] = k << 8;
       count = 1;
+      act[0] = 1u;
     }
   } else {
     const uint32_t* tm = p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, f.t_half, b, e) * p.mask_words;
     bool active = false;
-    for (int bb = 0; bb < f.num_blocks && !active; ++bb)
-      for (int w = 0; w < p.mask_words; ++w) active |= (tm[w] & f.block_masks[bb * p.mask_words + w]) != 0u;
+    for (int bb = 0; bb < f.num_blocks; ++bb) {
+      uint32_t hit = 0u;
+      for (int w = 0; w < p.mask_words; ++w) hit |= tm[w] & f.block_masks[bb * p.mask_words + w];
+      if (hit) {
+        act[bb >> 5] |= 1u << (bb & 31);
+        active = true;
+      }
+    }
     if (active) {
       const int m = items_for(f.kind, b, e, n);
       for (int j = 0; j < m; ++j) {
@@ -445,7 +453,12 @@ __global__ void k_plan(PlanArgs p) {
       }
     }
   }
-  if (!FILL) f.counts[k] = count;
+  if (!FILL) {
+    f.counts[k] = count;
+    // A cell whose children all fail keeps no item: no block is active there.
+#pragma unroll
+    for (int w = 0; w < 4; ++w) f.act[(size_t)w * p.num_cells + k] = count ? act[w] : 0u;
+  }
 }
 
 // In-place exclusive scan of counts[0, num_cells) into counts[0, num_cells]; one block per family.
@@ -493,25 +506,21 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   const int my_k = k0 + lane;
   float acc = semi_zero<SEMI>();
   float* buf = s_buf[w];
-  // Masked layers: this lane's target-cell mask, for the active-block test of each slot.
-  const uint32_t* tmask = nullptr;
-  if (a.masks != nullptr && my_k < kend) {
-    const int packed = a.cell_list[my_k];
-    const int sent = packed >> 8, b = packed & 255;
-    const int n = a.lengths[sent];
-    tmask = a.masks + (size_t)cell_abs(a.cell_offsets, sent, n, a.t_half, b, b + a.span) * a.mask_words;
-  }
   if (t < a.n_t) {
+    int cur_f = -1;
+    bool use = true, any_use = true;
     for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
       const int f = a.fam[e];
-      const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
-      bool use = true;
-      if (tmask != nullptr && a.slot_block[f] != nullptr) {
-        const uint32_t* bm = a.block_masks[f] + (size_t)a.slot_block[f][a.slot[e]] * a.mask_words;
-        uint32_t hit = 0;
-        for (int m = 0; m < a.mask_words; ++m) hit |= tmask[m] & bm[m];
-        use = hit != 0u;
+      if (f != cur_f && a.act[f] != nullptr) {
+        // Entries of a target are grouped by family, and all slots of the target in one family
+        // share its block: test it once per family, and skip the family if no lane needs it.
+        cur_f = f;
+        const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
+        use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_k] >> (blk & 31)) & 1u);
+        any_use = __ballot(use) != 0ull;
       }
+      if (!any_use) continue;
+      const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
       if (a.prefix[f] == nullptr) {
         // Split-major items: split j of cell k is item j * num_cells + k; lanes read consecutive
         // cells of one split row.
@@ -586,6 +595,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
     // setRootScores (CLParserUtils.scala:66-70) on the outside top root cell.
     v = (t_out == a.root) ? a.one : semi_zero<SEMI>();
   }
+  if (a.act[0] != nullptr && v == semi_zero<SEMI>() && !(a.root_init && b == 0 && e == n)) return;
   const int c = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
   a.chT[(size_t)c * a.cell_size + t_out] = v;
 }
@@ -1456,10 +1466,11 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
   // Under masks, plan the span first: per family, the counts/prefix (ncells + 1) and the item
   // list (at most the unpruned item count), all on the device.
   std::vector<const int32_t*> plan_counts(R.families.size(), nullptr), plan_items(R.families.size(), nullptr);
+  std::vector<const uint32_t*> plan_act(R.families.size(), nullptr);
   const uint32_t* masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words
                                     : nullptr;
   if (use_masks) {
-    std::vector<size_t> oc(R.families.size()), oi(R.families.size());
+    std::vector<size_t> oc(R.families.size()), oi(R.families.size()), oa(R.families.size());
     size_t total = 0;
     for (size_t fi = 0; fi < R.families.size(); ++fi) {
       const int fid = R.families[fi];
@@ -1472,6 +1483,8 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       }
       oc[fi] = total;
       total += (size_t)ncells + 1;
+      oa[fi] = total;
+      total += (size_t)4 * ncells;
       oi[fi] = total;
       total += (size_t)std::max<int64_t>(maxi, 1);
     }
@@ -1490,8 +1503,10 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       pf.block_masks = F.unary ? nullptr : F.d_block_masks;
       pf.counts = mp + oc[fi];
       pf.items = mp + oi[fi];
+      pf.act = (uint32_t*)(mp + oa[fi]);
       plan_counts[fi] = pf.counts;
       plan_items[fi] = pf.items;
+      plan_act[fi] = pf.act;
     }
     pa.masks = masks;
     pa.cell_offsets = B.d_cell_offsets;
@@ -1545,8 +1560,9 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       if (use_masks) {
         ra.prefix[fi] = plan_counts[fi] + k0;
         ra.uniform_count[fi] = 1;
-        ra.slot_block[fi] = F.unary ? nullptr : F.d_slot_block;
-        ra.block_masks[fi] = F.unary ? nullptr : F.d_block_masks;
+        ra.act[fi] = plan_act[fi] + k0;
+        ra.block_of_target[fi] = F.unary ? nullptr : F.d_block_of_target;
+        ra.act_stride = ncells;
       }
       const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
       ra.partial[fi] = base + off;
@@ -1559,8 +1575,6 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     const DevFamily& F0 = L.fam[R.families[0]];
     ra.chT = F0.t_half >= 2 ? outside : inside;
     ra.t_half = F0.t_half;
-    ra.masks = masks;
-    ra.mask_words = g.mask_words;
     ra.cell_offsets = B.d_cell_offsets;
     ra.lengths = B.d_lengths;
     ra.cell_list = B.d_cells + B.span_start[span] + k0;

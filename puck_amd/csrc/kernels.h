@@ -67,12 +67,13 @@ struct RedArgs {
   const int32_t* prefix[kMaxGroupFamilies];  // slice item prefix, null = uniform count
   int32_t uniform_count[kMaxGroupFamilies];
   int32_t pitch[kMaxGroupFamilies];
-  // Masked layers: a (cell, slot) pair contributes only when the slot's block is active in the
-  // target cell (its mask meets the block mask); inactive pairs were never computed.
-  const uint32_t* masks;  // null = unmasked
-  const int32_t* slot_block[kMaxGroupFamilies];
-  const uint32_t* block_masks[kMaxGroupFamilies];
-  int32_t mask_words;
+  // Masked layers: a (cell, target) pair takes a family's partials only when the target's block
+  // is active in the cell (act: the plan's active-block bits, 4 rows of act_stride words, slice-
+  // relative); inactive pairs were never computed. Zero results are not written (the chart half
+  // was filled with the semiring zero).
+  const uint32_t* act[kMaxGroupFamilies];  // null = unmasked
+  const int32_t* block_of_target[kMaxGroupFamilies];  // null = block 0
+  int32_t act_stride;
   int32_t num_cells, cell_size, n_t, t_half, span;
   int32_t root_init, root;
   float one;
@@ -84,6 +85,7 @@ struct PlanFam {
   const uint32_t* block_masks;
   int32_t* counts;  // num_cells + 1: counts, then their exclusive prefix
   int32_t* items;   // planned items, (cell << 8) | split
+  uint32_t* act;    // active-block bits per cell: 4 rows of num_cells words
 };
 struct PlanArgs {
   PlanFam fam[kMaxGroupFamilies];
