@@ -78,6 +78,95 @@ def profiled_traffic():
     return None
 
 
+def timed_parses(parser, words, offsets, steps):
+    """Warm-up parse, then `steps` parses; returns (device ms per parse, [(ms, apps, family)] of the
+    last parse)."""
+    parser.parse((words, offsets))
+    dev = 0.0
+    for _ in range(steps):
+        parser.parse((words, offsets))
+        dev += parser.stats()["device_ms"]
+    return dev / steps, parser.launches()
+
+
+def fine_roofline(launches):
+    """fp32 VALU roofline of the fine layer's inside binary rule launches (pruned rule-apps)."""
+    ms, apps, fam = launches
+    sel = (fam >= 16) & ((fam % 16) < 4)
+    t = float(ms[sel].sum()) / 1e3
+    ach = 3.0 * float(apps[sel].sum()) / t if t > 0 else 0.0
+    return {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": round(VALU_PEAK / 1e12, 2),
+            "unit": "TFLOP/s", "frac": round(ach / VALU_PEAK, 4), "kernel": "k_rules (fine inside binary, masked)",
+            "launches": int(sel.sum()), "rule_apps": float(apps[sel].sum())}
+
+
+def serve(puck, layers, pool, rate, seconds, seed):
+    """Open-loop Poisson arrivals at `rate` requests/s for `seconds`; returns throughput and latency."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.2) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    srv = puck.Server(layers, mem_bytes=96 << 30, max_inflight=3, max_batch=20000)
+    srv.stats(reset=True)
+    t0 = time.perf_counter()
+    for k, at in enumerate(arrivals):
+        while True:
+            dt = at - (time.perf_counter() - t0)
+            if dt <= 0:
+                break
+            if dt > 0.002:
+                time.sleep(dt - 0.001)
+        srv.submit(pool[k % len(pool)])
+    srv.flush()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    del srv
+    return {"rate": rate, "seconds": seconds, "requests": int(len(arrivals)),
+            "throughput": round(st["completed"] / wall, 1), "p50_ms": round(st["p50_ms"], 2),
+            "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2), "batches": int(st["batches"]),
+            "mean_batch": round(st["mean_batch"], 1)}
+
+
+def side_workloads(puck, g6v, prefix, args):
+    """Coarse-to-fine, mixed-length and serving workloads, measured beside the headline."""
+    out = {}
+    g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
+    words, offsets = g6v.sample(seed=3, n=args.c2f_sentences, mode=0, min_len=1, max_len=40)
+    lens = np.diff(offsets)
+    p = puck.CLParser([g1, g6v], device=0, mem_bytes=64 << 30)
+    ms, launches = timed_parses(p, words, offsets, 2)
+    out["c2f_viterbi"] = {
+        "workload": f"wsj_1 -> synthetic wsj_6-shape Viterbi, masks at -9, {len(lens)} PCFG sentences 1-40",
+        "value": round(len(lens) / (ms / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms, 2),
+        "mean_length": round(float(lens.mean()), 2), "roofline": fine_roofline(launches)}
+    print(f"[bench] c2f viterbi {out['c2f_viterbi']['value']} sentences/s", file=sys.stderr, flush=True)
+    g6r = puck.Grammar.load_text(prefix, puck.REAL)
+    pr = puck.CLParser([g1, g6r], device=0, mem_bytes=64 << 30)
+    ms, launches = timed_parses(pr, words, offsets, 2)
+    out["c2f_mbr"] = {
+        "workload": f"wsj_1 -> synthetic wsj_6-shape Real inside/outside + MBR, {len(lens)} PCFG sentences 1-40",
+        "value": round(len(lens) / (ms / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms, 2),
+        "roofline": fine_roofline(launches)}
+    print(f"[bench] c2f mbr {out['c2f_mbr']['value']} sentences/s", file=sys.stderr, flush=True)
+    del pr, g6r
+    mw, mo = g6v.sample(seed=4, n=args.mixed_sentences, mode=2, min_len=1, max_len=80)
+    ml = np.diff(mo)
+    ms, launches = timed_parses(p, mw, mo, 1)
+    out["mixed_lengths"] = {
+        "workload": f"c2f Viterbi, {len(ml)} sentences, 70% log-normal 1-40 + 30% uniform 41-80",
+        "value": round(len(ml) / (ms / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms, 2),
+        "mean_length": round(float(ml.mean()), 2), "roofline": fine_roofline(launches)}
+    print(f"[bench] mixed lengths {out['mixed_lengths']['value']} sentences/s", file=sys.stderr, flush=True)
+    del p
+    pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
+    pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
+    out["serving"] = []
+    for rate in (200, 1000, 5000):
+        r = serve(puck, [g1, g6v], pool, rate, args.serve_seconds, seed=rate)
+        out["serving"].append(r)
+        print(f"[bench] serving {r}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +178,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="sentences timed on the CPU oracle (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--check", type=int, default=8, help="sentences whose trees are checked against the oracle")
+    ap.add_argument("--side", type=int, default=1, help="also run the c2f, mixed-length and serving workloads")
+    ap.add_argument("--c2f-sentences", type=int, default=5000)
+    ap.add_argument("--mixed-sentences", type=int, default=2000)
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="seconds of Poisson arrivals per rate")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()
@@ -174,6 +267,8 @@ def main():
         k = min(args.check, args.cpu_sample)
         ok = all(np.array_equal(trees[i], ref_trees[tree_off[i]:tree_off[i + 1]]) for i in range(k))
         line["trees_match_oracle"] = f"{k}/{k}" if ok else "MISMATCH"
+    if args.side:
+        line.update(side_workloads(puck, g, prefix, args))
     print(json.dumps(line))
 
 

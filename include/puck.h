@@ -130,8 +130,9 @@ int32_t puck_debug_charts(puck_parser_t* p, const int32_t* words, const int32_t*
                           float* in_scale, float* out_scale, uint32_t* masks,
                           double* partition_scales, int64_t chart_cap);
 int32_t puck_parser_stats(const puck_parser_t* p, puck_stats_t* out);
-/* Per rule-kernel launch of the last call: duration (ms), rule applications, family id
- * (0-3 inside NN/NT/TN/TT, 4-7 outside-left, 8-11 outside-right, 12-15 unary). Returns the count. */
+/* Per rule-kernel launch of the last call: duration (ms), rule applications (under pruning masks:
+ * the rules of the active blocks per planned item), and family id + 16 x layer index (families
+ * 0-3 inside NN/NT/TN/TT, 4-7 outside-left, 8-11 outside-right, 12-15 unary). Returns the count. */
 int64_t puck_parser_launches(const puck_parser_t* p, float* ms, double* apps, int32_t* family, int64_t cap);
 
 /* ---- serving (replaces BatchFunctionAnnotatorService, AnnotatorService.scala:29-78) ----

@@ -307,7 +307,8 @@ class CLParser:
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
     def launches(self):
-        """Per rule-kernel launch of the last call: (ms float32[], rule_apps float64[], family int32[])."""
+        """Per rule-kernel launch of the last call: (ms float32[], rule_apps float64[], family int32[]);
+        family = family id + 16 x layer index."""
         n = lib().puck_parser_launches(self._h, None, None, None, 0)
         ms = np.zeros(max(n, 1), np.float32)
         apps = np.zeros(max(n, 1), np.float64)

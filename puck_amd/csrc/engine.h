@@ -75,6 +75,7 @@ struct DevFamily {
   int32_t* d_block_of_target = nullptr;
   uint32_t* d_block_masks = nullptr;
   double rule_apps_per_item = 0;  // rules in the family (for roofline accounting)
+  double* d_block_rules = nullptr;  // rules per block (per the whole family for unaries)
 };
 
 // Reduction group: families whose partials are combined into one target chart half.
@@ -169,9 +170,8 @@ class Parser {
   PassStats stats;
   const Layer& layer(int i) const { return *layers_[i]; }
   int32_t num_layers() const { return (int32_t)layers_.size(); }
-  // Execution mode: 0 = one launch per span and family (the reference schedule), 1 = the same
-  // launches recorded once per batch shape into a HIP graph and replayed.
-  int32_t mode = 0;
+  // Per-launch timing (HIP events around every rule launch); off for serving.
+  bool timing = true;
 
  private:
   struct Batch;
@@ -180,7 +180,9 @@ class Parser {
   int64_t mem_bytes_;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  struct TimedLaunch { hipEvent_t a, b; double apps; int32_t family; };
+  // apps_slot >= 0: the launch's rule applications are the plan's count in apps_[apps_slot].
+  struct TimedLaunch { hipEvent_t a, b; double apps; int32_t family; int32_t apps_slot; };
+  int32_t apps_next_ = 0;
   std::vector<hipEvent_t> ev_pool_;
   size_t ev_next_ = 0;
   std::vector<TimedLaunch> pending_;
@@ -196,7 +198,7 @@ class Parser {
   // Workspace (grown on demand).
   struct Buf { void* p = nullptr; size_t bytes = 0; };
   Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
-      pos_scale_, aux_, mplan_;
+      pos_scale_, aux_, mplan_, apps_;
   void ensure(Buf& b, size_t bytes);
 
   void run_batch(const int32_t* words, const int32_t* offsets, int32_t first, int32_t count,
@@ -208,7 +210,7 @@ class Parser {
   void outside_pass(Batch& b, int32_t li, bool use_masks);
   void launch_family(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks,
                      int32_t slice_cell0, int32_t slice_cells, float* partial_base,
-                     const int32_t* plan_counts, const int32_t* plan_items);
+                     const int32_t* plan_counts, const int32_t* plan_items, int32_t apps_slot);
   void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks);
   void timed_begin();
   void timed_end(double* acc);

@@ -454,6 +454,18 @@ __global__ void k_plan(PlanArgs p) {
     }
   }
   if (!FILL) {
+    if (count > 0) {
+      // Rule applications the reference semantics perform here: every rule of every active block
+      // for every item (CLParser.scala:899-901).
+      double rules = 0.0;
+      if (f.unary) {
+        rules = f.block_rules[0];
+      } else {
+        for (int bb = 0; bb < f.num_blocks; ++bb)
+          if ((act[bb >> 5] >> (bb & 31)) & 1u) rules += f.block_rules[bb];
+      }
+      atomicAdd(f.apps, rules * count);
+    }
     f.counts[k] = count;
     // A cell whose children all fail keeps no item: no block is active there.
 #pragma unroll
@@ -1188,6 +1200,9 @@ void Layer::build_family(int32_t id) {
   F.num_slots = (int32_t)F.h_slot_target.size();
   F.d_chunk_range = upload(F.h_chunk_range);
   F.d_slot_target = upload(F.h_slot_target);
+  std::vector<double> block_rules(std::max(1, F.unary ? 1 : F.num_blocks), 0.0);
+  for (auto& r : rules) block_rules[F.unary ? 0 : std::min(bot[r.t], (int)block_rules.size() - 1)] += 1.0;
+  F.d_block_rules = upload(block_rules);
   F.d_rules = upload(packed);
   F.d_wave_blocks = upload(wave_blocks);
   F.d_slot_block = upload(F.h_slot_block);
@@ -1265,7 +1280,7 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
 
 Parser::~Parser() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
-                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_})
+                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_})
     if (b->p) (void)hipFree(b->p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -1325,7 +1340,7 @@ bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
 
 void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks,
                            int32_t k0, int32_t kc, float* partial, const int32_t* plan_counts,
-                           const int32_t* plan_items) {
+                           const int32_t* plan_items, int32_t apps_slot) {
   Layer& L = *layers_[li];
   const DevFamily& F = L.fam[fid];
   const Grammar& g = L.g;
@@ -1391,8 +1406,12 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   a.row_stride = 64 * L.ipl + (L.ipl == 1 ? 1 : 2);
   dim3 grid((unsigned)((nitems + items_tile - 1) / items_tile), (unsigned)F.num_chunks);
   const size_t lds = (size_t)F.max_rows * a.row_stride * 4;
-  hipEvent_t t0 = next_event(), t1 = next_event();
-  HIP_CHECK(hipEventRecord(t0, stream_));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (timing) {
+    t0 = next_event();
+    t1 = next_event();
+    HIP_CHECK(hipEventRecord(t0, stream_));
+  }
   launch_semi(g.semiring, [&](auto semi) {
     constexpr int S = decltype(semi)::value;
     if (L.ipl == 2) {
@@ -1406,10 +1425,13 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
     }
   });
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipEventRecord(t1, stream_));
-  const double apps = (double)nitems * F.rule_apps_per_item;
-  pending_.push_back(TimedLaunch{t0, t1, apps, fid});
+  // Planned launches take their count from the plan (the first slice of a span carries the whole
+  // span's count); others count every rule for every item.
+  const double apps = plan_items != nullptr ? 0.0 : (double)nitems * F.rule_apps_per_item;
   stats.rule_apps += apps;
+  if (!timing) return;
+  HIP_CHECK(hipEventRecord(t1, stream_));
+  pending_.push_back(TimedLaunch{t0, t1, apps, fid + 16 * li, plan_items != nullptr ? apps_slot : -1});
   stats.rule_launches += 1;
 }
 
@@ -1432,14 +1454,23 @@ void Parser::mark_device_end() {
 
 void Parser::flush_timing() {
   HIP_CHECK(hipStreamSynchronize(stream_));
+  std::vector<double> plan_apps(apps_next_, 0.0);
+  if (apps_next_ > 0)
+    HIP_CHECK(hipMemcpy(plan_apps.data(), apps_.p, (size_t)apps_next_ * 8, hipMemcpyDeviceToHost));
   for (const TimedLaunch& t : pending_) {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+    double apps = t.apps;
+    if (t.apps_slot >= 0) {
+      apps = plan_apps[t.apps_slot];
+      stats.rule_apps += apps;
+    }
     stats.rule_kernel_ms += ms;
     stats.launch_ms.push_back(ms);
-    stats.launch_apps.push_back(t.apps);
+    stats.launch_apps.push_back(apps);
     stats.launch_family.push_back(t.family);
   }
+  apps_next_ = 0;
   for (const Phase& ph : phases_) {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, ph.a, ph.b));
@@ -1467,6 +1498,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
   // list (at most the unpruned item count), all on the device.
   std::vector<const int32_t*> plan_counts(R.families.size(), nullptr), plan_items(R.families.size(), nullptr);
   std::vector<const uint32_t*> plan_act(R.families.size(), nullptr);
+  std::vector<int32_t> apps_slot(R.families.size(), -1);
   const uint32_t* masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words
                                     : nullptr;
   if (use_masks) {
@@ -1504,6 +1536,10 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       pf.counts = mp + oc[fi];
       pf.items = mp + oi[fi];
       pf.act = (uint32_t*)(mp + oa[fi]);
+      pf.block_rules = F.d_block_rules;
+      apps_slot[fi] = apps_next_;
+      if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
+      pf.apps = (double*)apps_.p + apps_next_++;
       plan_counts[fi] = pf.counts;
       plan_items[fi] = pf.items;
       plan_act[fi] = pf.act;
@@ -1567,7 +1603,8 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
       ra.partial[fi] = base + off;
       ra.pitch[fi] = pitch;
-      launch_family(B, li, fid, span, use_masks, k0, kc, base + off, plan_counts[fi], plan_items[fi]);
+      launch_family(B, li, fid, span, use_masks, k0, kc, base + off, plan_counts[fi], plan_items[fi],
+                    k0 == 0 ? apps_slot[fi] : -1);
       off += (size_t)pitch * F.num_slots;
     }
     float* inside = (float*)inside_.p;
@@ -1728,6 +1765,13 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   ensure(pos_scale_, (size_t)B.total_len * 8);
   const size_t partial_bytes = std::max<size_t>((size_t)(mem_bytes_ * 0.3), (size_t)64 << 20);
   ensure(partial_, partial_bytes);
+  // Plan counters: one double per planned (layer, span, family group member) of the batch.
+  // (Four reduce groups per span and layer at most: inside binary and unary, outside binary and
+  // unary.)
+  const size_t apps_cap = (size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies;
+  ensure(apps_, apps_cap * 8);
+  hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)apps_.p, apps_cap * 2, 0.0f);
+  apps_next_ = 0;
 
   const int32_t nl = (int32_t)layers_.size();
   const int32_t last_layer = (what == 3) ? nl - 2 : nl - 1;

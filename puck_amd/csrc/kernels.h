@@ -86,6 +86,8 @@ struct PlanFam {
   int32_t* counts;  // num_cells + 1: counts, then their exclusive prefix
   int32_t* items;   // planned items, (cell << 8) | split
   uint32_t* act;    // active-block bits per cell: 4 rows of num_cells words
+  const double* block_rules;  // rules per block (binary) or the rule count (unary, one entry)
+  double* apps;     // accumulates the rule applications the plan implies
 };
 struct PlanArgs {
   PlanFam fam[kMaxGroupFamilies];

@@ -16,8 +16,10 @@ Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   for (auto& g : layers) max_cell = std::max(max_cell, g->cell_size);
   // Same split as Parser::make_batches: 30% of a slot's memory for two charts per layer.
   max_cells_ = std::max<int64_t>(1, (int64_t)(per_slot * 0.3) / ((int64_t)max_cell * 4));
-  for (int32_t i = 0; i < max_inflight; ++i)
+  for (int32_t i = 0; i < max_inflight; ++i) {
     parsers_.emplace_back(new Parser(layers, device, per_slot, items_per_lane));
+    parsers_.back()->timing = false;
+  }
   for (int32_t i = 0; i < max_inflight; ++i) threads_.emplace_back(&Server::worker, this, i);
 }
 
