@@ -256,7 +256,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   // over the blocks that none of its items allows; the cursor (cb, c_end) belongs to the furthest
   // group fetched so far. Otherwise the wave streams one range and counts its slots.
   const int nbr = a.num_block_ranges;
-  ConstI32* bt = (ConstI32*)a.wave_blocks + (size_t)(chunk * kWaves + wave) * (nbr + 2);
+  ConstI32* bt = (ConstI32*)a.wave_blocks + (size_t)(chunk * kWaves + wave) * (nbr + 2 + kPieceTableInts);
+  // This workgroup's piece of the stream: [p_lo, p_hi) in groups, first slot p_slot.
+  const int npieces = 1 << a.pieces_log2;
+  ConstI32* sp = bt + nbr + 2 + 2 * ((npieces - 1) + a.pieces_log2) + 2 * (int)blockIdx.z;
+  const int p_lo = sp[0], p_slot = sp[1], p_hi = sp[2];
   uint64_t un_lo = ~0ull, un_hi = ~0ull;  // block bits 0-63 and 64-127 of the tile's union
   if (SKIP && a.masks != nullptr) {
     un_lo = (uint64_t)__builtin_amdgcn_readfirstlane(s_union[0]) |
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     while (++cb < nbr) {
       const uint64_t word = (cb & 64) ? un_hi : un_lo;
       if (!((word >> (cb & 63)) & 1u)) continue;
-      const int b0 = bt[cb], b1 = bt[cb + 1];
+      const int b0 = max(bt[cb], p_lo), b1 = min(bt[cb + 1], p_hi);
       if (b0 < b1) {
         c_end = (uint32_t)b1 * GB;
         return (uint32_t)b0 * GB;
@@ -287,9 +291,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     if (gi == sentinel) return;
     g1 = advance(gi);
   } else {
-    gi = (uint32_t)bt[0] * GB;
-    g_end = (uint32_t)bt[1] * GB;
-    slot_ctr = (uint32_t)bt[2];
+    gi = (uint32_t)p_lo * GB;
+    g_end = (uint32_t)p_hi * GB;
+    slot_ctr = (uint32_t)p_slot;
     if (gi >= g_end) return;
     g1 = gi + GB;
   }
@@ -1170,8 +1174,45 @@ void Layer::build_family(int32_t id) {
           if (block_major || br == nbr - 1)
             while (stream.size() % kGroupRules) stream.push_back(null_rule());
         }
-        wave_blocks.push_back((int32_t)(packed.size() / gw + stream.size() / kGroupRules));
+        const int32_t wave_g0 = (int32_t)(packed.size() / gw);
+        const int32_t wave_len = (int32_t)(stream.size() / kGroupRules);
+        wave_blocks.push_back(wave_g0 + wave_len);
         wave_blocks.push_back(first_slot);
+        // Split tables: for P = 1, 2, 4, ..., kMaxPieces, P + 1 (group, first slot) points cutting
+        // the wave's stream into P pieces at places no slot spans (after a group whose second half
+        // ends a slot, at block starts, at the ends), each as close as possible to an even split.
+        {
+          std::vector<int32_t> cand{0};
+          std::vector<int32_t> slots_before(wave_len + 1, 0);
+          std::vector<char> is_block_start(wave_len + 1, 0);
+          for (int br = 0; br < nbr; ++br) {
+            const int32_t bs = wave_blocks[wave_blocks.size() - 2 - (nbr - br)] - wave_g0;
+            if (bs >= 0 && bs <= wave_len) is_block_start[bs] = 1;
+          }
+          for (int32_t gg = 0; gg < wave_len; ++gg) {
+            int32_t ends = 0;
+            for (int u = 0; u < kGroupRules; ++u) ends += stream[(size_t)gg * kGroupRules + u].tag != 0;
+            slots_before[gg + 1] = slots_before[gg] + ends;
+            const bool last_ends = stream[(size_t)gg * kGroupRules + kGroupRules - 1].tag != 0;
+            if (gg + 1 < wave_len && (last_ends || is_block_start[gg + 1])) cand.push_back(gg + 1);
+          }
+          if (wave_len > 0) cand.push_back(wave_len);
+          for (int pp = 1; pp <= kMaxPieces; pp *= 2) {
+            size_t ci = 0;
+            for (int k = 0; k <= pp; ++k) {
+              int32_t gpos = wave_len;
+              if (k == 0) {
+                gpos = 0;
+              } else if (k < pp) {
+                const int64_t target = ((int64_t)wave_len * k + pp / 2) / pp;
+                while (ci + 1 < cand.size() && cand[ci + 1] <= target) ++ci;
+                gpos = cand[ci];
+              }
+              wave_blocks.push_back(wave_g0 + gpos);
+              wave_blocks.push_back(first_slot + slots_before[gpos]);
+            }
+          }
+        }
         for (size_t g0 = 0; g0 < stream.size(); g0 += kGroupRules) {
           uint32_t words[kSkipGroupWords] = {};
           for (int u = 0; u < kGroupRules; ++u) {
@@ -1404,7 +1445,13 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   a.t_half = F.t_half;
   a.mask_words = g.mask_words;
   a.row_stride = 64 * L.ipl + (L.ipl == 1 ? 1 : 2);
-  dim3 grid((unsigned)((nitems + items_tile - 1) / items_tile), (unsigned)F.num_chunks);
+  // Few tiles: cut the rule streams into pieces so that the launch still fills the chip (about
+  // two workgroups per CU); large launches run whole streams.
+  const int64_t tiles = (nitems + items_tile - 1) / items_tile;
+  int plog = 0;
+  while (plog < kLog2MaxPieces && tiles * F.num_chunks * (2 << plog) <= 512) ++plog;
+  a.pieces_log2 = plog;
+  dim3 grid((unsigned)tiles, (unsigned)F.num_chunks, 1u << plog);
   const size_t lds = (size_t)F.max_rows * a.row_stride * 4;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (timing) {

@@ -22,6 +22,11 @@ constexpr int kGroupWords = 24;
 constexpr int kSkipGroupWords = 32;
 constexpr uint32_t kOffsetMask = (1u << 18) - 1;
 constexpr uint32_t kSlotEnd = 0x80000000u;
+// A launch may cut every wave's rule stream into P = 1, 2, 4, ..., kMaxPieces pieces (grid z),
+// at precomputed places no slot spans, so that few tiles still spread over the whole chip.
+constexpr int kMaxPieces = 16;
+constexpr int kLog2MaxPieces = 4;
+constexpr int kPieceTableInts = 2 * (2 * kMaxPieces + kLog2MaxPieces);  // sum over P of 2 (P + 1)
 constexpr int kReduceTargets = 16;    // reduce kernel: targets (waves) per workgroup
 
 // Arguments of the rule-family kernel (one launch = one family, one span, one cell slice).
@@ -40,7 +45,8 @@ struct FamArgs {
   const int32_t* prefix;
   const int32_t* plan_items;  // planned items of the span: (cell << 8) | split, null = all items
   const int32_t* chunk_range;
-  // Per (chunk, wave): num_block_ranges + 1 group starts, then the wave's first slot id.
+  // Per (chunk, wave): num_block_ranges + 1 group starts, the wave's first slot id, then the
+  // split tables: for P = 1, 2, 4, ..., P + 1 (group, first slot) pairs.
   const int32_t* wave_blocks;
   const uint32_t* rules;  // rule groups, kGroupWords dwords each
   const int32_t* block_of_target;
@@ -48,6 +54,7 @@ struct FamArgs {
   float* partial;
   int32_t num_cells, num_items, uniform_count, slice_cell0;
   int32_t num_block_ranges, sentinel_group;
+  int32_t pieces_log2;  // the launch cuts streams into 2^pieces_log2 pieces (grid z)
   int32_t span, kind, range_split, cell_size;
   int32_t a_half, b_half, t_half;
   int32_t mask_words, num_blocks, unary_gate;
