@@ -423,7 +423,7 @@ __global__ void k_plan(PlanArgs p) {
   const int sent = packed >> 8, b = packed & 255, e = b + p.span;
   const int n = p.lengths[sent];
   int count = 0;
-  int32_t* out = FILL ? f.items + f.counts[k] : nullptr;
+  int32_t* out = FILL ? f.items + f.pre[p.posof[k]] : nullptr;
   uint32_t act[4] = {0u, 0u, 0u, 0u};
   if (f.unary) {
     if (mask_any(p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, 1, b, e) * p.mask_words, p.mask_words)) {
@@ -474,18 +474,68 @@ __global__ void k_plan(PlanArgs p) {
     // A cell whose children all fail keeps no item: no block is active there.
 #pragma unroll
     for (int w = 0; w < 4; ++w) f.act[(size_t)w * p.num_cells + k] = count ? act[w] : 0u;
+    if ((int)blockIdx.y == p.key_fam) {
+      int key = kPlanBuckets - 1;
+      if (count > 0) {
+        key = 0;
+        for (int i = 0; i < 4; ++i) {
+          const int bb = p.key_blocks[i];
+          if (bb >= 0 && ((act[bb >> 5] >> (bb & 31)) & 1u)) key |= 1 << i;
+        }
+      }
+      p.keys[k] = key;
+    }
   }
 }
 
-// In-place exclusive scan of counts[0, num_cells) into counts[0, num_cells]; one block per family.
+// Stable counting sort of the slice's cells by key (one workgroup): perm and posof.
+__global__ __launch_bounds__(512) void k_plan_sort(PlanArgs p) {
+  constexpr int T = 512;
+  __shared__ int32_t s_a[kPlanBuckets * T];
+  __shared__ int32_t s_sum[T];
+  const int t = threadIdx.x;
+  const int per = (p.num_cells + T - 1) / T;
+  const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
+  for (int b = 0; b < kPlanBuckets; ++b) s_a[b * T + t] = 0;
+  for (int k = lo; k < hi; ++k) ++s_a[p.keys[k] * T + t];
+  __syncthreads();
+  // Exclusive scan of the bucket-major table: entry (b, t) becomes the sorted position of thread
+  // t's first cell of bucket b.
+  int32_t local = 0;
+  for (int i = 0; i < kPlanBuckets; ++i) local += s_a[t * kPlanBuckets + i];
+  s_sum[t] = local;
+  __syncthreads();
+  for (int d = 1; d < T; d <<= 1) {
+    const int32_t v = t >= d ? s_sum[t - d] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int32_t run = s_sum[t] - local;
+  for (int i = 0; i < kPlanBuckets; ++i) {
+    const int32_t v = s_a[t * kPlanBuckets + i];
+    s_a[t * kPlanBuckets + i] = run;
+    run += v;
+  }
+  __syncthreads();
+  for (int k = lo; k < hi; ++k) {
+    const int pos = s_a[p.keys[k] * T + t]++;
+    p.perm[pos] = k;
+    p.posof[k] = pos;
+  }
+}
+
+// Exclusive scan of the item counts in the sorted cell order into pre[0, num_cells]; one block per
+// family.
 __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
   __shared__ int32_t s_sum[1024];
-  int32_t* c = p.fam[blockIdx.y].counts;
+  const int32_t* c = p.fam[blockIdx.y].counts;
+  int32_t* pre = p.fam[blockIdx.y].pre;
   const int t = threadIdx.x;
   const int per = (p.num_cells + 1023) / 1024;
   const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
   int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += c[i];
+  for (int i = lo; i < hi; ++i) sum += c[p.perm[i]];
   s_sum[t] = sum;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {
@@ -496,11 +546,10 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
   }
   int32_t run = s_sum[t] - sum;  // exclusive prefix of this thread's range
   for (int i = lo; i < hi; ++i) {
-    const int32_t v = c[i];
-    c[i] = run;
-    run += v;
+    pre[i] = run;
+    run += c[p.perm[i]];
   }
-  if (t == 1023) c[p.num_cells] = s_sum[1023];
+  if (t == 1023) pre[p.num_cells] = s_sum[1023];
 }
 
 // Combine partials: target[cell][t] = (+) over (family, slot) of t over the cell's items.
@@ -519,7 +568,8 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   const int k0 = blockIdx.x * 64;
   const int kend = min(k0 + 64, a.num_cells);
   const int t = blockIdx.y * kReduceTargets + w;
-  const int my_k = k0 + lane;
+  const int my_k = k0 + lane;  // position in the visiting order
+  const int my_c = (a.perm != nullptr && my_k < kend) ? a.perm[my_k] : my_k;  // slice cell
   float acc = semi_zero<SEMI>();
   float* buf = s_buf[w];
   if (t < a.n_t) {
@@ -532,7 +582,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
         // share its block: test it once per family, and skip the family if no lane needs it.
         cur_f = f;
         const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
-        use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_k] >> (blk & 31)) & 1u);
+        use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
         any_use = __ballot(use) != 0ull;
       }
       if (!any_use) continue;
@@ -604,7 +654,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   const int k = k0 + kk;
   if (k >= a.num_cells || t_out >= a.n_t) return;
   float v = s_out[tt][kk];
-  const int packed = a.cell_list[k];
+  const int packed = a.cell_list[a.perm != nullptr ? a.perm[k] : k];
   const int sent = packed >> 8, b = packed & 255, e = b + a.span;
   const int n = a.lengths[sent];
   if (a.root_init && b == 0 && e == n) {
@@ -1243,6 +1293,14 @@ void Layer::build_family(int32_t id) {
   F.d_slot_target = upload(F.h_slot_target);
   std::vector<double> block_rules(std::max(1, F.unary ? 1 : F.num_blocks), 0.0);
   for (auto& r : rules) block_rules[F.unary ? 0 : std::min(bot[r.t], (int)block_rules.size() - 1)] += 1.0;
+  // The four blocks with most rules key the mask plan's cell order.
+  if (!F.unary && F.num_blocks > 1) {
+    std::vector<int32_t> ord(block_rules.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int32_t)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return block_rules[x] > block_rules[y]; });
+    ord.resize(std::min<size_t>(4, ord.size()));
+    F.h_key_blocks = ord;
+  }
   F.d_block_rules = upload(block_rules);
   F.d_rules = upload(packed);
   F.d_wave_blocks = upload(wave_blocks);
@@ -1401,11 +1459,11 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   a.cell_list = B.d_cells + B.span_start[span] + k0;
   a.num_cells = kc;
   int64_t nitems;
-  a.slice_cell0 = k0;
   a.plan_items = plan_items;
+  a.slice_cell0 = 0;  // planned item codes are slice-relative
   if (plan_items != nullptr) {
     // Planned slice: its item count is on the device; the grid covers the unpruned count.
-    a.prefix = plan_counts + k0;
+    a.prefix = plan_counts;
     a.uniform_count = 1;
     if (uniform_kind(F.kind)) {
       nitems = (int64_t)kc * (F.kind == 0 ? span - 1 : 1);
@@ -1534,84 +1592,34 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
   const Grammar& g = L.g;
   const int32_t ncells = B.span_count[span];
   if (ncells == 0) return;
-  // Bytes of partials per cell for each family of the group.
+  const size_t nfam = R.families.size();
+  // Items per cell for each family of the group, unpruned (an upper bound under masks).
   auto cell_items = [&](int fid, int k) -> int64_t {
     const DevFamily& F = L.fam[fid];
     if (uniform_kind(F.kind)) return F.kind == 0 ? span - 1 : 1;
     const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
     return B.h_prefix[po + k + 1] - B.h_prefix[po + k];
   };
-  // Under masks, plan the span first: per family, the counts/prefix (ncells + 1) and the item
-  // list (at most the unpruned item count), all on the device.
-  std::vector<const int32_t*> plan_counts(R.families.size(), nullptr), plan_items(R.families.size(), nullptr);
-  std::vector<const uint32_t*> plan_act(R.families.size(), nullptr);
-  std::vector<int32_t> apps_slot(R.families.size(), -1);
+  auto slice_items = [&](int fid, int k0, int kc) -> int64_t {
+    const DevFamily& F = L.fam[fid];
+    if (uniform_kind(F.kind)) return (int64_t)kc * (F.kind == 0 ? span - 1 : 1);
+    const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+    return B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
+  };
   const uint32_t* masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words
                                     : nullptr;
-  if (use_masks) {
-    std::vector<size_t> oc(R.families.size()), oi(R.families.size()), oa(R.families.size());
-    size_t total = 0;
-    for (size_t fi = 0; fi < R.families.size(); ++fi) {
-      const int fid = R.families[fi];
-      int64_t maxi = 0;
-      if (uniform_kind(L.fam[fid].kind)) {
-        maxi = (int64_t)ncells * (L.fam[fid].kind == 0 ? span - 1 : 1);
-      } else {
-        const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
-        maxi = B.h_prefix[po + ncells] - B.h_prefix[po];
-      }
-      oc[fi] = total;
-      total += (size_t)ncells + 1;
-      oa[fi] = total;
-      total += (size_t)4 * ncells;
-      oi[fi] = total;
-      total += (size_t)std::max<int64_t>(maxi, 1);
-    }
-    ensure(mplan_, total * 4);
-    PlanArgs pa{};
-    int32_t* mp = (int32_t*)mplan_.p;
-    for (size_t fi = 0; fi < R.families.size(); ++fi) {
-      const DevFamily& F = L.fam[R.families[fi]];
-      PlanFam& pf = pa.fam[fi];
-      pf.kind = F.kind;
-      pf.unary = F.unary;
-      pf.a_half = F.a_half;
-      pf.b_half = F.b_half;
-      pf.t_half = F.t_half;
-      pf.num_blocks = F.unary ? 0 : F.num_blocks;
-      pf.block_masks = F.unary ? nullptr : F.d_block_masks;
-      pf.counts = mp + oc[fi];
-      pf.items = mp + oi[fi];
-      pf.act = (uint32_t*)(mp + oa[fi]);
-      pf.block_rules = F.d_block_rules;
-      apps_slot[fi] = apps_next_;
-      if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
-      pf.apps = (double*)apps_.p + apps_next_++;
-      plan_counts[fi] = pf.counts;
-      plan_items[fi] = pf.items;
-      plan_act[fi] = pf.act;
-    }
-    pa.masks = masks;
-    pa.cell_offsets = B.d_cell_offsets;
-    pa.lengths = B.d_lengths;
-    pa.cell_list = B.d_cells + B.span_start[span];
-    pa.num_cells = ncells;
-    pa.span = span;
-    pa.mask_words = g.mask_words;
-    const unsigned nf = (unsigned)R.families.size();
-    const dim3 pg((ncells + 255) / 256, nf);
-    hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, stream_, pa);
-    hipLaunchKernelGGL(k_plan_scan, dim3(1, nf), dim3(1024), 0, stream_, pa);
-    hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, stream_, pa);
-    HIP_CHECK(hipGetLastError());
-  }
-  // Each family's partial region is num_slots x pitch floats, pitch = items rounded up to 256.
+  // Under masks the group's cells are visited in one order keyed by the family with most rules.
+  int key_fam = 0;
+  for (size_t fi = 1; fi < nfam; ++fi)
+    if (L.fam[R.families[fi]].num_rules > L.fam[R.families[key_fam]].num_rules) key_fam = (int)fi;
+
+  // Slices of cells whose partials fit the partial buffer (num_slots x pitch floats per family,
+  // pitch = items rounded up to 256).
   const int64_t budget_floats = (int64_t)(partial_.bytes / 4);
   int64_t fixed = 0;
   for (int fid : R.families) fixed += 256 * (int64_t)L.fam[fid].num_slots;
   int32_t k0 = 0;
   while (k0 < ncells) {
-    // Grow the slice while the partial buffer holds it.
     int32_t k1 = k0;
     int64_t used = 0;
     while (k1 < ncells) {
@@ -1623,35 +1631,97 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     }
     if (used + fixed > budget_floats) throw std::runtime_error("partial buffer too small for one cell");
     const int32_t kc = k1 - k0;
+    const int32_t* cells = B.d_cells + B.span_start[span] + k0;
+
+    // Mask plan of the slice (device): per family counts, active-block bits and the rule
+    // applications; one cell order for the group; per family the prefix and item list in it.
+    std::vector<PlanFam> pfs(nfam);
+    std::vector<int32_t> apps_slot(nfam, -1);
+    int32_t* d_perm = nullptr;
+    if (use_masks) {
+      size_t total = 3 * (size_t)kc;  // keys, perm, posof
+      std::vector<size_t> oc(nfam), op(nfam), oi(nfam), oa(nfam);
+      for (size_t fi = 0; fi < nfam; ++fi) {
+        oc[fi] = total;
+        total += (size_t)kc;
+        op[fi] = total;
+        total += (size_t)kc + 1;
+        oa[fi] = total;
+        total += (size_t)4 * kc;
+        oi[fi] = total;
+        total += (size_t)std::max<int64_t>(slice_items(R.families[fi], k0, kc), 1);
+      }
+      ensure(mplan_, total * 4);
+      int32_t* mp = (int32_t*)mplan_.p;
+      PlanArgs pa{};
+      for (size_t fi = 0; fi < nfam; ++fi) {
+        const DevFamily& F = L.fam[R.families[fi]];
+        PlanFam& pf = pa.fam[fi];
+        pf.kind = F.kind;
+        pf.unary = F.unary;
+        pf.a_half = F.a_half;
+        pf.b_half = F.b_half;
+        pf.t_half = F.t_half;
+        pf.num_blocks = F.unary ? 0 : F.num_blocks;
+        pf.block_masks = F.unary ? nullptr : F.d_block_masks;
+        pf.counts = mp + oc[fi];
+        pf.pre = mp + op[fi];
+        pf.items = mp + oi[fi];
+        pf.act = (uint32_t*)(mp + oa[fi]);
+        pf.block_rules = F.d_block_rules;
+        if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
+        apps_slot[fi] = apps_next_;
+        pf.apps = (double*)apps_.p + apps_next_++;
+        pfs[fi] = pf;
+      }
+      pa.keys = mp;
+      pa.perm = mp + kc;
+      pa.posof = mp + 2 * (size_t)kc;
+      d_perm = pa.perm;
+      pa.masks = masks;
+      pa.cell_offsets = B.d_cell_offsets;
+      pa.lengths = B.d_lengths;
+      pa.cell_list = cells;
+      pa.num_cells = kc;
+      pa.span = span;
+      pa.mask_words = g.mask_words;
+      pa.key_fam = key_fam;
+      const DevFamily& KF = L.fam[R.families[key_fam]];
+      for (int i = 0; i < 4; ++i) pa.key_blocks[i] = i < (int)KF.h_key_blocks.size() ? KF.h_key_blocks[i] : -1;
+      const dim3 pg((kc + 255) / 256, (unsigned)nfam);
+      hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, stream_, pa);
+      hipLaunchKernelGGL(k_plan_sort, dim3(1), dim3(512), 0, stream_, pa);
+      hipLaunchKernelGGL(k_plan_scan, dim3(1, (unsigned)nfam), dim3(1024), 0, stream_, pa);
+      hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, stream_, pa);
+      HIP_CHECK(hipGetLastError());
+    }
+
     RedArgs ra{};
     float* base = (float*)partial_.p;
     size_t off = 0;
-    for (size_t fi = 0; fi < R.families.size(); ++fi) {
+    for (size_t fi = 0; fi < nfam; ++fi) {
       const int fid = R.families[fi];
       const DevFamily& F = L.fam[fid];
-      int64_t nitems;
-      if (uniform_kind(F.kind)) {
+      const int64_t nitems = slice_items(fid, k0, kc);
+      if (use_masks) {
+        ra.prefix[fi] = pfs[fi].pre;
+        ra.uniform_count[fi] = 1;
+        ra.act[fi] = pfs[fi].act;
+        ra.block_of_target[fi] = F.unary ? nullptr : F.d_block_of_target;
+        ra.act_stride = kc;
+      } else if (uniform_kind(F.kind)) {
         ra.prefix[fi] = nullptr;
         ra.uniform_count[fi] = F.kind == 0 ? span - 1 : 1;
-        nitems = (int64_t)kc * ra.uniform_count[fi];
       } else {
         const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
         ra.prefix[fi] = B.d_prefix + po + k0;
         ra.uniform_count[fi] = 1;
-        nitems = B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
-      }
-      if (use_masks) {
-        ra.prefix[fi] = plan_counts[fi] + k0;
-        ra.uniform_count[fi] = 1;
-        ra.act[fi] = plan_act[fi] + k0;
-        ra.block_of_target[fi] = F.unary ? nullptr : F.d_block_of_target;
-        ra.act_stride = ncells;
       }
       const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
       ra.partial[fi] = base + off;
       ra.pitch[fi] = pitch;
-      launch_family(B, li, fid, span, use_masks, k0, kc, base + off, plan_counts[fi], plan_items[fi],
-                    k0 == 0 ? apps_slot[fi] : -1);
+      launch_family(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? pfs[fi].pre : nullptr,
+                    use_masks ? pfs[fi].items : nullptr, apps_slot[fi]);
       off += (size_t)pitch * F.num_slots;
     }
     float* inside = (float*)inside_.p;
@@ -1659,9 +1729,10 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     const DevFamily& F0 = L.fam[R.families[0]];
     ra.chT = F0.t_half >= 2 ? outside : inside;
     ra.t_half = F0.t_half;
+    ra.perm = d_perm;
     ra.cell_offsets = B.d_cell_offsets;
     ra.lengths = B.d_lengths;
-    ra.cell_list = B.d_cells + B.span_start[span] + k0;
+    ra.cell_list = cells;
     ra.off = R.d_off;
     ra.fam = R.d_fam;
     ra.slot = R.d_slot;
@@ -1814,8 +1885,8 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   ensure(partial_, partial_bytes);
   // Plan counters: one double per planned (layer, span, family group member) of the batch.
   // (Four reduce groups per span and layer at most: inside binary and unary, outside binary and
-  // unary.)
-  const size_t apps_cap = (size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies;
+  // unary; room for eight partial-buffer slices each.)
+  const size_t apps_cap = (size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies * 8;
   ensure(apps_, apps_cap * 8);
   hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)apps_.p, apps_cap * 2, 0.0f);
   apps_next_ = 0;
