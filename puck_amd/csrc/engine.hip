@@ -572,7 +572,49 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   const int my_c = (a.perm != nullptr && my_k < kend) ? a.perm[my_k] : my_k;  // slice cell
   float acc = semi_zero<SEMI>();
   float* buf = s_buf[w];
-  if (t < a.n_t) {
+  if (t < a.n_t && a.act[0] != nullptr) {
+    // Masked (planned) groups: a cell's items are contiguous in the plan's order and few, so each
+    // lane reads its own items straight from the partial rows, four slots of a family at a time.
+    const int e_end = a.off[t + 1];
+    for (int e = a.off[t]; e < e_end;) {
+      const int f = a.fam[e];
+      int run = e + 1;
+      while (run < e_end && a.fam[run] == f) ++run;
+      const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
+      const bool use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
+      if (__ballot(use) != 0ull && use) {
+        const int32_t* pre = a.prefix[f];
+        const int lo = pre[my_k], hi = pre[my_k + 1];
+        const size_t pitch = (size_t)a.pitch[f];
+        const float* base = a.partial[f];
+        int q = e;
+        for (; q + 4 <= run; q += 4) {
+          const float* p0 = base + (size_t)a.slot[q] * pitch;
+          const float* p1 = base + (size_t)a.slot[q + 1] * pitch;
+          const float* p2 = base + (size_t)a.slot[q + 2] * pitch;
+          const float* p3 = base + (size_t)a.slot[q + 3] * pitch;
+          for (int j = lo; j < hi; ++j) {
+            const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
+            if (SEMI == kViterbi) {
+              acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
+            } else {
+              acc += v0;
+              acc += v1;
+              acc += v2;
+              acc += v3;
+            }
+          }
+        }
+        for (; q < run; ++q) {
+          const float* p0 = base + (size_t)a.slot[q] * pitch;
+          for (int j = lo; j < hi; ++j) {
+            if (SEMI == kViterbi) acc = fmaxf(acc, p0[j]); else acc += p0[j];
+          }
+        }
+      }
+      e = run;
+    }
+  } else if (t < a.n_t) {
     int cur_f = -1;
     bool use = true, any_use = true;
     for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
