@@ -76,7 +76,7 @@ struct DevFamily {
   uint32_t* d_block_masks = nullptr;
   double rule_apps_per_item = 0;  // rules in the family (for roofline accounting)
   double* d_block_rules = nullptr;  // rules per block (per the whole family for unaries)
-  std::vector<int32_t> h_key_blocks;  // up to four blocks with most rules (mask plan sort key)
+  std::vector<int32_t> h_key_blocks;  // the blocks with most rules, most first (mask plan sort key)
 };
 
 // Reduction group: families whose partials are combined into one target chart half.

@@ -423,7 +423,7 @@ __global__ void k_plan(PlanArgs p) {
   const int sent = packed >> 8, b = packed & 255, e = b + p.span;
   const int n = p.lengths[sent];
   int count = 0;
-  int32_t* out = FILL ? f.items + f.pre[p.posof[k]] : nullptr;
+  int32_t* out = FILL ? f.items + f.pre[f.posof[k]] : nullptr;
   uint32_t act[4] = {0u, 0u, 0u, 0u};
   if (f.unary) {
     if (mask_any(p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, 1, b, e) * p.mask_words, p.mask_words)) {
@@ -474,55 +474,67 @@ __global__ void k_plan(PlanArgs p) {
     // A cell whose children all fail keeps no item: no block is active there.
 #pragma unroll
     for (int w = 0; w < 4; ++w) f.act[(size_t)w * p.num_cells + k] = count ? act[w] : 0u;
-    if ((int)blockIdx.y == p.key_fam) {
-      int key = kPlanBuckets - 1;
-      if (count > 0) {
-        key = 0;
-        for (int i = 0; i < 4; ++i) {
-          const int bb = p.key_blocks[i];
-          if (bb >= 0 && ((act[bb >> 5] >> (bb & 31)) & 1u)) key |= 1 << i;
-        }
+    // Sort key: the family's largest blocks active here, the largest as the most significant bit.
+    int key = 0;
+    if (count > 0) {
+#pragma unroll
+      for (int i = 0; i < kPlanKeyBlocks; ++i) {
+        const int bb = f.key_blocks[i];
+        if (bb >= 0 && ((act[bb >> 5] >> (bb & 31)) & 1u)) key |= 1 << (kPlanKeyBlocks - 1 - i);
       }
-      p.keys[k] = key;
     }
+    f.keys[k] = key;
   }
 }
 
-// Stable counting sort of the slice's cells by key (one workgroup): perm and posof.
+// Stable sort of the slice's cells by the family's key (one workgroup per family): two counting-
+// sort passes over 4-bit digits, low digit first. Pass one writes (key << 24 | cell) in low-digit
+// order into posof, pass two scatters cells by the high digit into perm, then posof is rebuilt
+// as perm's inverse.
 __global__ __launch_bounds__(512) void k_plan_sort(PlanArgs p) {
-  constexpr int T = 512;
-  __shared__ int32_t s_a[kPlanBuckets * T];
+  constexpr int T = 512, D = 16;
+  static_assert(kPlanKeyBlocks == 8, "two 4-bit digits");
+  __shared__ int32_t s_a[D * T];
   __shared__ int32_t s_sum[T];
+  const PlanFam& f = p.fam[blockIdx.y];
   const int t = threadIdx.x;
   const int per = (p.num_cells + T - 1) / T;
   const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
-  for (int b = 0; b < kPlanBuckets; ++b) s_a[b * T + t] = 0;
-  for (int k = lo; k < hi; ++k) ++s_a[p.keys[k] * T + t];
-  __syncthreads();
-  // Exclusive scan of the bucket-major table: entry (b, t) becomes the sorted position of thread
-  // t's first cell of bucket b.
-  int32_t local = 0;
-  for (int i = 0; i < kPlanBuckets; ++i) local += s_a[t * kPlanBuckets + i];
-  s_sum[t] = local;
-  __syncthreads();
-  for (int d = 1; d < T; d <<= 1) {
-    const int32_t v = t >= d ? s_sum[t - d] : 0;
+  int32_t* tmp = f.posof;
+  for (int pass = 0; pass < 2; ++pass) {
+    auto digit = [&](int i) -> int {
+      return pass == 0 ? (f.keys[i] & (D - 1)) : ((tmp[i] >> 28) & (D - 1));
+    };
+    for (int b = 0; b < D; ++b) s_a[b * T + t] = 0;
+    for (int i = lo; i < hi; ++i) ++s_a[digit(i) * T + t];
     __syncthreads();
-    s_sum[t] += v;
+    // Exclusive scan of the digit-major table: entry (d, t) becomes the sorted position of thread
+    // t's first cell with digit d.
+    int32_t local = 0;
+    for (int i = 0; i < D; ++i) local += s_a[t * D + i];
+    s_sum[t] = local;
+    __syncthreads();
+    for (int d = 1; d < T; d <<= 1) {
+      const int32_t v = t >= d ? s_sum[t - d] : 0;
+      __syncthreads();
+      s_sum[t] += v;
+      __syncthreads();
+    }
+    int32_t run = s_sum[t] - local;
+    for (int i = 0; i < D; ++i) {
+      const int32_t v = s_a[t * D + i];
+      s_a[t * D + i] = run;
+      run += v;
+    }
+    __syncthreads();
+    if (pass == 0) {
+      for (int k = lo; k < hi; ++k) tmp[s_a[digit(k) * T + t]++] = (f.keys[k] << 24) | k;
+    } else {
+      for (int i = lo; i < hi; ++i) f.perm[s_a[digit(i) * T + t]++] = tmp[i] & 0xffffff;
+    }
     __syncthreads();
   }
-  int32_t run = s_sum[t] - local;
-  for (int i = 0; i < kPlanBuckets; ++i) {
-    const int32_t v = s_a[t * kPlanBuckets + i];
-    s_a[t * kPlanBuckets + i] = run;
-    run += v;
-  }
-  __syncthreads();
-  for (int k = lo; k < hi; ++k) {
-    const int pos = s_a[p.keys[k] * T + t]++;
-    p.perm[pos] = k;
-    p.posof[k] = pos;
-  }
+  for (int i = lo; i < hi; ++i) f.posof[f.perm[i]] = i;
 }
 
 // Exclusive scan of the item counts in the sorted cell order into pre[0, num_cells]; one block per
@@ -530,12 +542,13 @@ __global__ __launch_bounds__(512) void k_plan_sort(PlanArgs p) {
 __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
   __shared__ int32_t s_sum[1024];
   const int32_t* c = p.fam[blockIdx.y].counts;
+  const int32_t* perm = p.fam[blockIdx.y].perm;
   int32_t* pre = p.fam[blockIdx.y].pre;
   const int t = threadIdx.x;
   const int per = (p.num_cells + 1023) / 1024;
   const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
   int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += c[p.perm[i]];
+  for (int i = lo; i < hi; ++i) sum += c[perm[i]];
   s_sum[t] = sum;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {
@@ -547,7 +560,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
   int32_t run = s_sum[t] - sum;  // exclusive prefix of this thread's range
   for (int i = lo; i < hi; ++i) {
     pre[i] = run;
-    run += c[p.perm[i]];
+    run += c[perm[i]];
   }
   if (t == 1023) pre[p.num_cells] = s_sum[1023];
 }
@@ -584,7 +597,8 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
       const bool use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
       if (__ballot(use) != 0ull && use) {
         const int32_t* pre = a.prefix[f];
-        const int lo = pre[my_k], hi = pre[my_k + 1];
+        const int pos = a.posof[f] != nullptr ? a.posof[f][my_c] : my_k;
+        const int lo = pre[pos], hi = pre[pos + 1];
         const size_t pitch = (size_t)a.pitch[f];
         const float* base = a.partial[f];
         int q = e;
@@ -1227,6 +1241,8 @@ void Layer::build_family(int32_t id) {
       const int nbr = F.num_block_ranges;
       // of_wave[w][b]: slots of block range b streamed by wave w.
       std::vector<std::vector<std::vector<size_t>>> of_wave(kWaves, std::vector<std::vector<size_t>>(nbr));
+      // Block-major streams are balanced per block (a tile may run any subset of the blocks), ties
+      // going to the wave with the least load over all blocks; otherwise over the whole stream.
       std::vector<int64_t> load(kWaves, 0);
       for (int br = 0; br < nbr; ++br) {
         std::vector<size_t> order;
@@ -1235,10 +1251,15 @@ void Layer::build_family(int32_t id) {
         std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
           return slots[x].second - slots[x].first > slots[y].second - slots[y].first;
         });
+        std::vector<int64_t> bload(kWaves, 0);
         // Per slot the flush costs about two rule groups; count it in the balance.
         for (size_t k : order) {
-          const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-          load[w] += (int64_t)padded(slots[k].second - slots[k].first) + 2 * kRuleUnroll;
+          int w = 0;
+          for (int v = 1; v < kWaves; ++v)
+            if (std::make_pair(bload[v], load[v]) < std::make_pair(bload[w], load[w])) w = v;
+          const int64_t c = (int64_t)padded(slots[k].second - slots[k].first) + 2 * kRuleUnroll;
+          load[w] += c;
+          if (block_major) bload[w] += c;
           of_wave[w][br].push_back(k);
         }
       }
@@ -1335,12 +1356,12 @@ void Layer::build_family(int32_t id) {
   F.d_slot_target = upload(F.h_slot_target);
   std::vector<double> block_rules(std::max(1, F.unary ? 1 : F.num_blocks), 0.0);
   for (auto& r : rules) block_rules[F.unary ? 0 : std::min(bot[r.t], (int)block_rules.size() - 1)] += 1.0;
-  // The four blocks with most rules key the mask plan's cell order.
+  // The blocks with most rules key the mask plan's cell order.
   if (!F.unary && F.num_blocks > 1) {
     std::vector<int32_t> ord(block_rules.size());
     for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int32_t)i;
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return block_rules[x] > block_rules[y]; });
-    ord.resize(std::min<size_t>(4, ord.size()));
+    ord.resize(std::min<size_t>(kPlanKeyBlocks, ord.size()));
     F.h_key_blocks = ord;
   }
   F.d_block_rules = upload(block_rules);
@@ -1650,7 +1671,8 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
   };
   const uint32_t* masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words
                                     : nullptr;
-  // Under masks the group's cells are visited in one order keyed by the family with most rules.
+  // Under masks every family orders the cells by its own key; the reduce visits them in the order
+  // of the family with most rules.
   int key_fam = 0;
   for (size_t fi = 1; fi < nfam; ++fi)
     if (L.fam[R.families[fi]].num_rules > L.fam[R.families[key_fam]].num_rules) key_fam = (int)fi;
@@ -1676,14 +1698,17 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     const int32_t* cells = B.d_cells + B.span_start[span] + k0;
 
     // Mask plan of the slice (device): per family counts, active-block bits and the rule
-    // applications; one cell order for the group; per family the prefix and item list in it.
+    // applications, the family's cell order, and the prefix and item list in it.
     std::vector<PlanFam> pfs(nfam);
     std::vector<int32_t> apps_slot(nfam, -1);
     int32_t* d_perm = nullptr;
     if (use_masks) {
-      size_t total = 3 * (size_t)kc;  // keys, perm, posof
-      std::vector<size_t> oc(nfam), op(nfam), oi(nfam), oa(nfam);
+      if (kc >= (1 << 24)) throw std::runtime_error("mask plan slice exceeds 2^24 cells");
+      size_t total = 0;
+      std::vector<size_t> oc(nfam), op(nfam), oi(nfam), oa(nfam), ok(nfam);
       for (size_t fi = 0; fi < nfam; ++fi) {
+        ok[fi] = total;  // keys, perm, posof
+        total += 3 * (size_t)kc;
         oc[fi] = total;
         total += (size_t)kc;
         op[fi] = total;
@@ -1710,16 +1735,18 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
         pf.pre = mp + op[fi];
         pf.items = mp + oi[fi];
         pf.act = (uint32_t*)(mp + oa[fi]);
+        pf.keys = mp + ok[fi];
+        pf.perm = mp + ok[fi] + kc;
+        pf.posof = mp + ok[fi] + 2 * (size_t)kc;
+        for (int i = 0; i < kPlanKeyBlocks; ++i)
+          pf.key_blocks[i] = i < (int)F.h_key_blocks.size() ? F.h_key_blocks[i] : -1;
         pf.block_rules = F.d_block_rules;
         if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
         apps_slot[fi] = apps_next_;
         pf.apps = (double*)apps_.p + apps_next_++;
         pfs[fi] = pf;
       }
-      pa.keys = mp;
-      pa.perm = mp + kc;
-      pa.posof = mp + 2 * (size_t)kc;
-      d_perm = pa.perm;
+      d_perm = pa.fam[key_fam].perm;
       pa.masks = masks;
       pa.cell_offsets = B.d_cell_offsets;
       pa.lengths = B.d_lengths;
@@ -1727,12 +1754,9 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       pa.num_cells = kc;
       pa.span = span;
       pa.mask_words = g.mask_words;
-      pa.key_fam = key_fam;
-      const DevFamily& KF = L.fam[R.families[key_fam]];
-      for (int i = 0; i < 4; ++i) pa.key_blocks[i] = i < (int)KF.h_key_blocks.size() ? KF.h_key_blocks[i] : -1;
       const dim3 pg((kc + 255) / 256, (unsigned)nfam);
       hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, stream_, pa);
-      hipLaunchKernelGGL(k_plan_sort, dim3(1), dim3(512), 0, stream_, pa);
+      hipLaunchKernelGGL(k_plan_sort, dim3(1, (unsigned)nfam), dim3(512), 0, stream_, pa);
       hipLaunchKernelGGL(k_plan_scan, dim3(1, (unsigned)nfam), dim3(1024), 0, stream_, pa);
       hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, stream_, pa);
       HIP_CHECK(hipGetLastError());
@@ -1747,6 +1771,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       const int64_t nitems = slice_items(fid, k0, kc);
       if (use_masks) {
         ra.prefix[fi] = pfs[fi].pre;
+        ra.posof[fi] = pfs[fi].posof;
         ra.uniform_count[fi] = 1;
         ra.act[fi] = pfs[fi].act;
         ra.block_of_target[fi] = F.unary ? nullptr : F.d_block_of_target;

@@ -81,26 +81,34 @@ struct RedArgs {
   const uint32_t* act[kMaxGroupFamilies];  // null = unmasked
   const int32_t* block_of_target[kMaxGroupFamilies];  // null = block 0
   int32_t act_stride;
-  // Planned groups visit cells in the plan's sorted order: position -> cell (null = identity).
+  // Planned groups visit cells in the order of the group's largest family (perm: position -> cell,
+  // null = identity); every family finds a cell's items at its own position of the cell (posof).
   const int32_t* perm;
+  const int32_t* posof[kMaxGroupFamilies];
   int32_t num_cells, cell_size, n_t, t_half, span;
   int32_t root_init, root;
   float one;
 };
 
-// Mask plan of one slice of a span for the families of a reduce group. Cells are put in one order
-// for the whole group: a stable counting sort by which of the key family's four largest blocks the
-// cell allows (kPlanBuckets - 1 = no item), so that tiles group cells with similar active blocks.
-constexpr int kPlanBuckets = 17;
+// Mask plan of one slice of a span for the families of a reduce group. Each family puts the cells
+// in its own order: a stable radix sort by which of the family's kPlanKeyBlocks largest blocks
+// the cell allows (the largest block is the most significant bit), so that a tile's items share
+// their active blocks and the rule kernel's walk over the union of its items' blocks wastes
+// little.
+constexpr int kPlanKeyBlocks = 8;
 struct PlanFam {
   int32_t kind, unary, a_half, b_half, t_half, num_blocks;
   const uint32_t* block_masks;
   int32_t* counts;  // num_cells: items per cell
-  int32_t* pre;     // num_cells + 1: item prefix in the sorted cell order
-  int32_t* items;   // planned items in the sorted cell order, (cell << 8) | split
+  int32_t* pre;     // num_cells + 1: item prefix in the family's cell order
+  int32_t* items;   // planned items in the family's cell order, (cell << 8) | split
   uint32_t* act;    // active-block bits per cell: 4 rows of num_cells words
+  int32_t* keys;    // num_cells: sort key of each cell
+  int32_t* perm;    // num_cells: sorted position -> cell
+  int32_t* posof;   // num_cells: cell -> sorted position
   const double* block_rules;  // rules per block (binary) or the rule count (unary, one entry)
   double* apps;     // accumulates the rule applications the plan implies
+  int32_t key_blocks[kPlanKeyBlocks];  // the family's largest blocks, largest first (-1: none)
 };
 struct PlanArgs {
   PlanFam fam[kMaxGroupFamilies];
@@ -108,11 +116,7 @@ struct PlanArgs {
   const int32_t* cell_offsets;
   const int32_t* lengths;
   const int32_t* cell_list;  // cells of the slice
-  int32_t* keys;   // num_cells: sort bucket of each cell
-  int32_t* perm;   // num_cells: sorted position -> cell
-  int32_t* posof;  // num_cells: cell -> sorted position
-  int32_t num_cells, span, mask_words, key_fam;
-  int32_t key_blocks[4];  // the key family's four largest blocks (-1: none)
+  int32_t num_cells, span, mask_words;
 };
 
 }  // namespace puck
