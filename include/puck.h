@@ -107,8 +107,25 @@ int32_t puck_sample_sentences(const puck_grammar_t* g, uint64_t seed, int32_t n,
                               int32_t* offsets_out);
 
 /* ---- parser (CLParser over coarse-to-fine layers; the last layer decides Viterbi vs MBR) ---- */
+/* Flags of puck_parser_create. */
+enum puck_parser_flags {
+  PUCK_FLAG_ONE_ITEM_PER_LANE = 1, /* rule kernels compute one (cell, split) item per lane, not two */
+  PUCK_FLAG_PERSISTENT = 2         /* rule families of a span share one persistent work-queue launch */
+};
+/* new CLParser(data, maxAllocSize) (CLParser.scala:46-48): mem_bytes bounds the device workspace,
+ * as the reference's --mem option bounds WorkSpace.allocate (WorkSpace.scala:91-120). */
 int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device,
                            int64_t mem_bytes, int32_t flags, puck_parser_t** out);
+/* Launch shape of the rule kernels (the reference's equivalent knobs are the work-group shape and
+ * rule-segment size fixed at kernel generation, JavaFriendlyGenRuleMultiply.scala:57-65): plain
+ * launches cut every wave's rule stream into pieces while tiles x chunks x 2 x pieces <=
+ * piece_target; persistent launches run queue_wgs_per_cu workgroups per CU. */
+typedef struct {
+  int32_t persistent;
+  int32_t piece_target;     /* >= 1, default 512 */
+  int32_t queue_wgs_per_cu; /* 1..8, default 1 */
+} puck_launch_config_t;
+int32_t puck_parser_configure(puck_parser_t* p, const puck_launch_config_t* cfg);
 void puck_parser_free(puck_parser_t* p);
 /* Sentences are word ids: sentence i is words[offsets[i] .. offsets[i+1]). */
 int32_t puck_parse_viterbi(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,

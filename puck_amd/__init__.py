@@ -54,6 +54,14 @@ class Stats(ctypes.Structure):
                 ("device_ms", ctypes.c_double)]
 
 
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [("persistent", ctypes.c_int32), ("piece_target", ctypes.c_int32),
+                ("queue_wgs_per_cu", ctypes.c_int32)]
+
+
+FLAG_ONE_ITEM_PER_LANE = 1
+FLAG_PERSISTENT = 2
+
 # Exported symbols and their signatures: (name, restype, argtypes).
 _P = ctypes.c_void_p
 _I32P = ctypes.POINTER(ctypes.c_int32)
@@ -77,6 +85,7 @@ SIGNATURES = [
     ("puck_parser_create", ctypes.c_int32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
     ("puck_parser_free", None, [_P]),
+    ("puck_parser_configure", ctypes.c_int32, [_P, ctypes.POINTER(LaunchConfig)]),
     ("puck_parse_viterbi", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(TreeNode), ctypes.c_int64]),
     ("puck_parse_mbr", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(MbrNode), ctypes.c_int64]),
     ("puck_partitions", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, _F32P]),
@@ -219,13 +228,18 @@ def synth_wsj6_cached(seed=6, cache_dir=None):
 class CLParser:
     """Coarse-to-fine CKY parser over grammar layers on one GPU (CLParser.scala:46-952)."""
 
-    def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2):
+    def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2, persistent=False):
         self.layers = list(layers)
         arr = (_P * len(self.layers))(*[g._h for g in self.layers])
         h = _P()
-        flags = 1 if items_per_lane == 1 else 0
+        flags = (FLAG_ONE_ITEM_PER_LANE if items_per_lane == 1 else 0) | (FLAG_PERSISTENT if persistent else 0)
         _check(lib().puck_parser_create(arr, len(self.layers), device, mem_bytes, flags, ctypes.byref(h)))
         self._h = h
+
+    def configure(self, persistent, piece_target=512, queue_wgs_per_cu=1):
+        """Launch shape of the rule kernels (see puck_parser_configure in include/puck.h)."""
+        cfg = LaunchConfig(int(bool(persistent)), int(piece_target), int(queue_wgs_per_cu))
+        _check(lib().puck_parser_configure(self._h, ctypes.byref(cfg)))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -2,6 +2,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -17,6 +18,7 @@ struct puck_grammar {
 
 struct puck_parser {
   std::unique_ptr<puck::Parser> p;
+  mutable std::mutex mu;  // serialises calls on one handle (the reference's synchronized methods)
 };
 
 struct puck_server {
@@ -219,15 +221,28 @@ int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers
       if (!layers[i]) return set_error(PUCK_ERR_ARG, "null grammar layer");
       gs.push_back(layers[i]->g);
     }
-    const int32_t ipl = (flags & 1) ? 1 : 2;
+    const int32_t ipl = (flags & PUCK_FLAG_ONE_ITEM_PER_LANE) ? 1 : 2;
     auto* h = new puck_parser;
     try {
       h->p.reset(new puck::Parser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, ipl));
+      h->p->persistent = (flags & PUCK_FLAG_PERSISTENT) != 0;
     } catch (...) {
       delete h;
       throw;
     }
     *out = h;
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_parser_configure(puck_parser_t* p, const puck_launch_config_t* cfg) {
+  return guarded([&]() -> int32_t {
+    if (!p || !cfg) return set_error(PUCK_ERR_ARG, "bad argument");
+    if (cfg->piece_target < 1 || cfg->queue_wgs_per_cu < 1 || cfg->queue_wgs_per_cu > 8)
+      return set_error(PUCK_ERR_ARG, "piece_target must be >= 1 and queue_wgs_per_cu in [1, 8]");
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->p->persistent = cfg->persistent != 0;
+    p->p->set_launch_shape(cfg->piece_target, cfg->queue_wgs_per_cu);
     return PUCK_OK;
   });
 }
@@ -240,6 +255,7 @@ int32_t puck_parse_viterbi(puck_parser_t* h, const int32_t* words, const int32_t
     if (!h || !words || !offsets || !nodes || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
     if (nodes_needed(offsets, n) > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
     std::vector<std::vector<puck::TreeNode>> trees;
+    std::lock_guard<std::mutex> lk(h->mu);
     h->p->parse(words, offsets, n, &trees, nullptr);
     int64_t o = 0;
     for (auto& t : trees) {
@@ -256,6 +272,7 @@ int32_t puck_parse_mbr(puck_parser_t* h, const int32_t* words, const int32_t* of
     if (!h || !words || !offsets || !nodes || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
     if (nodes_needed(offsets, n) > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
     std::vector<std::vector<puck::MbrNode>> trees;
+    std::lock_guard<std::mutex> lk(h->mu);
     h->p->parse(words, offsets, n, nullptr, &trees);
     int64_t o = 0;
     for (auto& t : trees) {
@@ -269,6 +286,7 @@ int32_t puck_parse_mbr(puck_parser_t* h, const int32_t* words, const int32_t* of
 int32_t puck_partitions(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n, float* out) {
   return guarded([&]() -> int32_t {
     if (!h || !words || !offsets || !out || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(h->mu);
     h->p->partitions(words, offsets, n, out);
     return PUCK_OK;
   });
@@ -277,6 +295,7 @@ int32_t puck_partitions(puck_parser_t* h, const int32_t* words, const int32_t* o
 int32_t puck_inside_outside(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n) {
   return guarded([&]() -> int32_t {
     if (!h || !words || !offsets || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(h->mu);
     h->p->inside_outside(words, offsets, n);
     return PUCK_OK;
   });
@@ -288,6 +307,7 @@ int32_t puck_masks(puck_parser_t* h, const int32_t* words, const int32_t* offset
     if (!h || !words || !offsets || !out || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
     std::vector<uint32_t> m;
     std::vector<int32_t> co;
+    std::lock_guard<std::mutex> lk(h->mu);
     h->p->masks(words, offsets, n, &m, &co);
     if ((int64_t)m.size() > cap) return set_error(PUCK_ERR_ARG, "mask buffer too small");
     std::memcpy(out, m.data(), m.size() * 4);
@@ -302,6 +322,7 @@ int32_t puck_debug_charts(puck_parser_t* h, const int32_t* words, const int32_t*
   return guarded([&]() -> int32_t {
     if (!h || !words || !offsets || n < 1) return set_error(PUCK_ERR_ARG, "bad argument");
     puck::Parser::Charts c;
+    std::lock_guard<std::mutex> lk(h->mu);
     h->p->debug_charts(words, offsets, n, layer, with_outside != 0, &c);
     if ((int64_t)c.inside.size() > chart_cap) return set_error(PUCK_ERR_ARG, "chart buffer too small");
     if (inside) std::memcpy(inside, c.inside.data(), c.inside.size() * 4);
@@ -318,6 +339,7 @@ int32_t puck_debug_charts(puck_parser_t* h, const int32_t* words, const int32_t*
 
 int32_t puck_parser_stats(const puck_parser_t* h, puck_stats_t* out) {
   if (!h || !out) return set_error(PUCK_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(h->mu);
   const puck::PassStats& s = h->p->stats;
   out->rule_kernel_ms = s.rule_kernel_ms;
   out->reduce_ms = s.reduce_ms;
@@ -331,6 +353,7 @@ int32_t puck_parser_stats(const puck_parser_t* h, puck_stats_t* out) {
 
 int64_t puck_parser_launches(const puck_parser_t* h, float* ms, double* apps, int32_t* family, int64_t cap) {
   if (!h) return set_error(PUCK_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(h->mu);
   const puck::PassStats& s = h->p->stats;
   const int64_t n = (int64_t)s.launch_ms.size();
   for (int64_t i = 0; i < n && i < cap; ++i) {

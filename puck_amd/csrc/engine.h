@@ -16,6 +16,8 @@
 
 namespace puck {
 
+struct FamArgs;
+
 // One node of a Viterbi tree, pre-order, 2n-1 nodes per sentence (CLViterbi.scala:378).
 struct TreeNode {
   int32_t top_sym;  // nonterminal id of the unary top, -1 if the node has no unary layer
@@ -173,6 +175,18 @@ class Parser {
   int32_t num_layers() const { return (int32_t)layers_.size(); }
   // Per-launch timing (HIP events around every rule launch); off for serving.
   bool timing = true;
+  // Persistent form: the rule families of a span share one k_rules_q launch over a device queue
+  // of work units (otherwise one k_rules launch per family and span).
+  bool persistent = false;
+  // Launch shape knobs (autotuned): a plain launch cuts rule streams into pieces while
+  // tiles x chunks x 2 x pieces stays within piece_target_; a persistent launch runs
+  // queue_wgs_per_cu_ workgroups per CU.
+  void set_launch_shape(int32_t piece_target, int32_t queue_wgs_per_cu) {
+    piece_target_ = piece_target;
+    queue_wgs_per_cu_ = queue_wgs_per_cu;
+  }
+  int32_t piece_target_ = 512;
+  int32_t queue_wgs_per_cu_ = 1;
 
  private:
   struct Batch;
@@ -181,8 +195,9 @@ class Parser {
   int64_t mem_bytes_;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  // apps_slot >= 0: the launch's rule applications are the plan's count in apps_[apps_slot].
-  struct TimedLaunch { hipEvent_t a, b; double apps; int32_t family; int32_t apps_slot; };
+  // apps_slot >= 0: the launch's rule applications add the plan counts in
+  // apps_[apps_slot, apps_slot + apps_count).
+  struct TimedLaunch { hipEvent_t a, b; double apps; int32_t family; int32_t apps_slot; int32_t apps_count; };
   int32_t apps_next_ = 0;
   std::vector<hipEvent_t> ev_pool_;
   size_t ev_next_ = 0;
@@ -199,7 +214,9 @@ class Parser {
   // Workspace (grown on demand).
   struct Buf { void* p = nullptr; size_t bytes = 0; };
   Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
-      pos_scale_, aux_, mplan_, apps_;
+      pos_scale_, aux_, mplan_, apps_, queue_;
+  int32_t queue_next_ = 0;
+  int32_t num_cus_ = 256;
   void ensure(Buf& b, size_t bytes);
 
   void run_batch(const int32_t* words, const int32_t* offsets, int32_t first, int32_t count,
@@ -209,9 +226,16 @@ class Parser {
   std::vector<std::pair<int32_t, int32_t>> make_batches(const int32_t* offsets, int32_t n);
   void inside_pass(Batch& b, int32_t li, bool use_masks);
   void outside_pass(Batch& b, int32_t li, bool use_masks);
+  bool family_args(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks, int32_t k0,
+                   int32_t kc, float* partial, const int32_t* plan_counts, const int32_t* plan_items,
+                   FamArgs* out, dim3* grid, size_t* lds);
   void launch_family(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks,
                      int32_t slice_cell0, int32_t slice_cells, float* partial_base,
                      const int32_t* plan_counts, const int32_t* plan_items, int32_t apps_slot);
+  void launch_queue(Batch& b, int32_t li, const std::vector<int32_t>& fids, int32_t span, bool use_masks,
+                    int32_t k0, int32_t kc, const std::vector<float*>& partials,
+                    const std::vector<const int32_t*>& plan_counts,
+                    const std::vector<const int32_t*>& plan_items, const std::vector<int32_t>& apps_slots);
   void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks);
   void timed_begin();
   void timed_end(double* acc);

@@ -98,18 +98,30 @@ __device__ inline float semi_zero() {
 // with buffer loads. Each slot writes one partial value per item; the reduce kernel combines
 // partials over splits, chunks and families. Rule semantics per SimpleGenRuleMultiply.java:151
 // (parent = semiring_mad(parent, left (x) right, q)) and :213-216 (Real: *= scale).
+// Per-tile bookkeeping of the rule kernels in static LDS (the operand rows use dynamic LDS).
+template <int ITEMS>
+struct RulesLds {
+  int32_t cell_a[ITEMS], cell_b[ITEMS];
+  uint32_t gate[4][ITEMS];  // 128 block bits per item
+  float scale[ITEMS];
+  uint32_t uni[4];          // blocks active in any item of the tile
+};
+
+// One work unit of a rule family: tile `tile` of ITEMS items x operand chunk `chunk` x piece
+// `piece` of every wave's rule stream. Every thread of the workgroup calls it with the same
+// arguments; it contains workgroup barriers before the rule loop only.
 template <int IPL, int SEMI, bool UNARY, bool SKIP>
-__global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
+__device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk, int piece, float* smem,
+                                           RulesLds<64 * IPL>& sh) {
   constexpr int ITEMS = 64 * IPL;
-  extern __shared__ __align__(16) float smem[];
-  __shared__ int32_t s_cell_a[ITEMS], s_cell_b[ITEMS];
-  __shared__ uint32_t s_gate[4][ITEMS];  // 128 block bits per item
-  __shared__ float s_scale[ITEMS];
-  __shared__ uint32_t s_union[4];  // blocks active in any item of the tile
+  int32_t* s_cell_a = sh.cell_a;
+  int32_t* s_cell_b = sh.cell_b;
+  uint32_t (*s_gate)[ITEMS] = sh.gate;
+  float* s_scale = sh.scale;
+  uint32_t* s_union = sh.uni;
   const float zero = semi_zero<SEMI>();
   const int tid = threadIdx.x;
-  const int tile0 = blockIdx.x * ITEMS;
-  const int chunk = blockIdx.y;
+  const int tile0 = tile * ITEMS;
   // Items of the slice: [item_begin, item_begin + num_items). Prefix-counted and planned slices
   // read their bounds on the device (the grid covers the largest possible count).
   int item_begin = 0, num_items = a.num_items;
@@ -259,7 +271,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   ConstI32* bt = (ConstI32*)a.wave_blocks + (size_t)(chunk * kWaves + wave) * (nbr + 2 + kPieceTableInts);
   // This workgroup's piece of the stream: [p_lo, p_hi) in groups, first slot p_slot.
   const int npieces = 1 << a.pieces_log2;
-  ConstI32* sp = bt + nbr + 2 + 2 * ((npieces - 1) + a.pieces_log2) + 2 * (int)blockIdx.z;
+  ConstI32* sp = bt + nbr + 2 + 2 * ((npieces - 1) + a.pieces_log2) + 2 * piece;
   const int p_lo = sp[0], p_slot = sp[1], p_hi = sp[2];
   uint64_t un_lo = ~0ull, un_hi = ~0ull;  // block bits 0-63 and 64-127 of the tile's union
   if (SKIP && a.masks != nullptr) {
@@ -406,6 +418,54 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
     if (done()) break;
     step(qB, sB, tB0, tB1, oA, tA0, tA1, oB, qA, sA, a1, b1, a0, b0);
     if (done()) break;
+  }
+}
+
+// Plain form: one launch per rule family and span, grid (tiles, chunks, pieces).
+template <int IPL, int SEMI, bool UNARY, bool SKIP>
+__global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
+  extern __shared__ __align__(16) float smem[];
+  __shared__ RulesLds<64 * IPL> sh;
+  rules_unit<IPL, SEMI, UNARY, SKIP>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem, sh);
+}
+
+// Persistent form: the rule families of one span share one launch of about one workgroup per CU.
+// Work units (family, tile, chunk, piece) form a queue, family-major and tile-fastest; each
+// workgroup pops the next unit from a device counter (zeroed before the launch) until the queue
+// is empty. Planned families take their item counts, hence their tile counts, from the device, so
+// the launch needs no host knowledge of the pruned work.
+template <int IPL, int SEMI, bool UNARY, bool SKIP>
+__global__ __launch_bounds__(kBlockThreads) void k_rules_q(QueueArgs q) {
+  constexpr int ITEMS = 64 * IPL;
+  extern __shared__ __align__(16) float smem[];
+  __shared__ RulesLds<ITEMS> sh;
+  __shared__ int32_t s_unit;
+  // Tiles of family f (planned families read their item count on the device).
+  auto tiles_of = [&](int f) {
+    const FamArgs& a = q.fam[f];
+    const int items = a.prefix != nullptr ? a.prefix[a.num_cells] - a.prefix[0] : a.num_items;
+    return (items + ITEMS - 1) / ITEMS;
+  };
+  int total = 0;
+  for (int f = 0; f < q.nfam; ++f) total += tiles_of(f) * q.fam[f].num_chunks * (1 << q.fam[f].pieces_log2);
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = atomicAdd(q.counter, 1);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    __syncthreads();  // every thread has read s_unit before thread 0 writes the next one
+    if (u >= total) break;
+    int f = 0, r = u, tiles = tiles_of(0);
+    for (;;) {
+      const int n = tiles * q.fam[f].num_chunks * (1 << q.fam[f].pieces_log2);
+      if (r < n || f + 1 == q.nfam) break;
+      r -= n;
+      tiles = tiles_of(++f);
+    }
+    const int tile = r % tiles;
+    const int cp = r / tiles;
+    const int chunk = cp % q.fam[f].num_chunks;
+    const int piece = cp / q.fam[f].num_chunks;
+    rules_unit<IPL, SEMI, UNARY, SKIP>(q.fam[f], tile, chunk, piece, smem, sh);
   }
 }
 
@@ -1438,11 +1498,21 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
                          (const void*)k_rules<1, kViterbi, false, true>, (const void*)k_rules<1, kReal, false, false>,
                          (const void*)k_rules<1, kReal, true, false>, (const void*)k_rules<1, kReal, false, true>})
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  for (const void* fn : {(const void*)k_rules_q<2, kViterbi, false, false>, (const void*)k_rules_q<2, kViterbi, true, false>,
+                         (const void*)k_rules_q<2, kViterbi, false, true>, (const void*)k_rules_q<2, kReal, false, false>,
+                         (const void*)k_rules_q<2, kReal, true, false>, (const void*)k_rules_q<2, kReal, false, true>,
+                         (const void*)k_rules_q<1, kViterbi, false, false>, (const void*)k_rules_q<1, kViterbi, true, false>,
+                         (const void*)k_rules_q<1, kViterbi, false, true>, (const void*)k_rules_q<1, kReal, false, false>,
+                         (const void*)k_rules_q<1, kReal, true, false>, (const void*)k_rules_q<1, kReal, false, true>})
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  int cus = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+  num_cus_ = std::max(1, cus);
 }
 
 Parser::~Parser() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
-                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_})
+                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_})
     if (b->p) (void)hipFree(b->p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
@@ -1500,9 +1570,9 @@ namespace {
 bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
 }  // namespace
 
-void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks,
-                           int32_t k0, int32_t kc, float* partial, const int32_t* plan_counts,
-                           const int32_t* plan_items, int32_t apps_slot) {
+bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks, int32_t k0,
+                         int32_t kc, float* partial, const int32_t* plan_counts, const int32_t* plan_items,
+                         FamArgs* out, dim3* grid, size_t* lds) {
   Layer& L = *layers_[li];
   const DevFamily& F = L.fam[fid];
   const Grammar& g = L.g;
@@ -1544,7 +1614,7 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
     nitems = B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
     a.uniform_count = 1;
   }
-  if (nitems <= 0) return;
+  if (nitems <= 0) return false;
   a.num_items = (int32_t)nitems;
   a.chunk_range = F.d_chunk_range;
   a.wave_blocks = F.d_wave_blocks;
@@ -1566,21 +1636,35 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   a.t_half = F.t_half;
   a.mask_words = g.mask_words;
   a.row_stride = 64 * L.ipl + (L.ipl == 1 ? 1 : 2);
+  a.num_chunks = F.num_chunks;
   // Few tiles: cut the rule streams into pieces so that the launch still fills the chip (about
   // two workgroups per CU); large launches run whole streams.
   const int64_t tiles = (nitems + items_tile - 1) / items_tile;
   int plog = 0;
-  while (plog < kLog2MaxPieces && tiles * F.num_chunks * (2 << plog) <= 512) ++plog;
+  while (plog < kLog2MaxPieces && tiles * F.num_chunks * (2 << plog) <= piece_target_) ++plog;
   a.pieces_log2 = plog;
-  dim3 grid((unsigned)tiles, (unsigned)F.num_chunks, 1u << plog);
-  const size_t lds = (size_t)F.max_rows * a.row_stride * 4;
+  *grid = dim3((unsigned)tiles, (unsigned)F.num_chunks, 1u << plog);
+  *lds = (size_t)F.max_rows * a.row_stride * 4;
+  *out = a;
+  return true;
+}
+
+void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks,
+                           int32_t k0, int32_t kc, float* partial, const int32_t* plan_counts,
+                           const int32_t* plan_items, int32_t apps_slot) {
+  Layer& L = *layers_[li];
+  const DevFamily& F = L.fam[fid];
+  FamArgs a;
+  dim3 grid;
+  size_t lds;
+  if (!family_args(B, li, fid, span, use_masks, k0, kc, partial, plan_counts, plan_items, &a, &grid, &lds)) return;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (timing) {
     t0 = next_event();
     t1 = next_event();
     HIP_CHECK(hipEventRecord(t0, stream_));
   }
-  launch_semi(g.semiring, [&](auto semi) {
+  launch_semi(L.g.semiring, [&](auto semi) {
     constexpr int S = decltype(semi)::value;
     if (L.ipl == 2) {
       if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, a);
@@ -1595,11 +1679,74 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   HIP_CHECK(hipGetLastError());
   // Planned launches take their count from the plan (the first slice of a span carries the whole
   // span's count); others count every rule for every item.
-  const double apps = plan_items != nullptr ? 0.0 : (double)nitems * F.rule_apps_per_item;
+  const double apps = plan_items != nullptr ? 0.0 : (double)a.num_items * F.rule_apps_per_item;
   stats.rule_apps += apps;
   if (!timing) return;
   HIP_CHECK(hipEventRecord(t1, stream_));
-  pending_.push_back(TimedLaunch{t0, t1, apps, fid + 16 * li, plan_items != nullptr ? apps_slot : -1});
+  pending_.push_back(TimedLaunch{t0, t1, apps, fid + 16 * li, plan_items != nullptr ? apps_slot : -1, 1});
+  stats.rule_launches += 1;
+}
+
+// Persistent form of a reduce group's rule families: families sharing a layout (unary or binary,
+// block-major or not) run as one k_rules_q launch over a device queue of work units.
+void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids, int32_t span,
+                          bool use_masks, int32_t k0, int32_t kc, const std::vector<float*>& partials,
+                          const std::vector<const int32_t*>& plan_counts,
+                          const std::vector<const int32_t*>& plan_items, const std::vector<int32_t>& apps_slots) {
+  Layer& L = *layers_[li];
+  QueueArgs q{};
+  size_t lds = 0;
+  int64_t units = 0;
+  double apps = 0;
+  int32_t first_slot = -1, nslots = 0;
+  for (size_t i = 0; i < fids.size(); ++i) {
+    // Plan counters of the families (planned ones only), consecutive by construction.
+    if (apps_slots[i] >= 0) {
+      if (first_slot < 0) first_slot = apps_slots[i];
+      if (apps_slots[i] != first_slot + nslots) throw std::logic_error("plan counters of a queue launch not contiguous");
+      ++nslots;
+    }
+    dim3 grid;
+    size_t l = 0;
+    FamArgs a;
+    if (!family_args(B, li, fids[i], span, use_masks, k0, kc, partials[i], plan_counts[i], plan_items[i], &a,
+                     &grid, &l))
+      continue;
+    if (q.nfam == kMaxQueueFamilies) throw std::logic_error("too many families in one queue launch");
+    q.fam[q.nfam++] = a;
+    lds = std::max(lds, l);
+    units += (int64_t)grid.x * grid.y * grid.z;
+    if (plan_items[i] == nullptr) apps += (double)a.num_items * L.fam[fids[i]].rule_apps_per_item;
+  }
+  if (q.nfam == 0) return;
+  if (units >= (int64_t)1 << 31) throw std::runtime_error("rule work queue exceeds 2^31 units");
+  if ((size_t)(queue_next_ + 1) * 4 > queue_.bytes) throw std::runtime_error("rule queue counters exhausted");
+  q.counter = (int32_t*)queue_.p + queue_next_++;
+  const DevFamily& F0 = L.fam[fids[0]];
+  const dim3 grid((unsigned)std::min<int64_t>(units, (int64_t)num_cus_ * queue_wgs_per_cu_));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (timing) {
+    t0 = next_event();
+    t1 = next_event();
+    HIP_CHECK(hipEventRecord(t0, stream_));
+  }
+  launch_semi(L.g.semiring, [&](auto semi) {
+    constexpr int S = decltype(semi)::value;
+    if (L.ipl == 2) {
+      if (F0.unary) hipLaunchKernelGGL((k_rules_q<2, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<2, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else hipLaunchKernelGGL((k_rules_q<2, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, q);
+    } else {
+      if (F0.unary) hipLaunchKernelGGL((k_rules_q<1, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<1, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else hipLaunchKernelGGL((k_rules_q<1, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, q);
+    }
+  });
+  HIP_CHECK(hipGetLastError());
+  stats.rule_apps += apps;
+  if (!timing) return;
+  HIP_CHECK(hipEventRecord(t1, stream_));
+  pending_.push_back(TimedLaunch{t0, t1, apps, fids[0] + 16 * li, first_slot, nslots});
   stats.rule_launches += 1;
 }
 
@@ -1629,9 +1776,9 @@ void Parser::flush_timing() {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
     double apps = t.apps;
-    if (t.apps_slot >= 0) {
-      apps = plan_apps[t.apps_slot];
-      stats.rule_apps += apps;
+    for (int32_t k = 0; t.apps_slot >= 0 && k < t.apps_count; ++k) {
+      apps += plan_apps[t.apps_slot + k];
+      stats.rule_apps += plan_apps[t.apps_slot + k];
     }
     stats.rule_kernel_ms += ms;
     stats.launch_ms.push_back(ms);
@@ -1765,6 +1912,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     RedArgs ra{};
     float* base = (float*)partial_.p;
     size_t off = 0;
+    std::vector<float*> fam_partial(nfam);
     for (size_t fi = 0; fi < nfam; ++fi) {
       const int fid = R.families[fi];
       const DevFamily& F = L.fam[fid];
@@ -1786,10 +1934,31 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       }
       const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
       ra.partial[fi] = base + off;
+      fam_partial[fi] = base + off;
       ra.pitch[fi] = pitch;
-      launch_family(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? pfs[fi].pre : nullptr,
-                    use_masks ? pfs[fi].items : nullptr, apps_slot[fi]);
+      if (!persistent)
+        launch_family(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? pfs[fi].pre : nullptr,
+                      use_masks ? pfs[fi].items : nullptr, apps_slot[fi]);
       off += (size_t)pitch * F.num_slots;
+    }
+    if (persistent) {
+      // One queue launch per layout among the group's families (unary, block-major, plain).
+      for (int layout = 0; layout < 3; ++layout) {
+        std::vector<int32_t> fids, slots;
+        std::vector<float*> parts;
+        std::vector<const int32_t*> pcs, pis;
+        for (size_t fi = 0; fi < nfam; ++fi) {
+          const DevFamily& F = L.fam[R.families[fi]];
+          const int lay = F.unary ? 0 : (F.skip_layout ? 1 : 2);
+          if (lay != layout) continue;
+          fids.push_back(R.families[fi]);
+          parts.push_back(fam_partial[fi]);
+          pcs.push_back(use_masks ? pfs[fi].pre : nullptr);
+          pis.push_back(use_masks ? pfs[fi].items : nullptr);
+          slots.push_back(apps_slot[fi]);
+        }
+        if (!fids.empty()) launch_queue(B, li, fids, span, use_masks, k0, kc, parts, pcs, pis, slots);
+      }
     }
     float* inside = (float*)inside_.p;
     float* outside = (float*)outside_.p;
@@ -1957,6 +2126,14 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   ensure(apps_, apps_cap * 8);
   hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)apps_.p, apps_cap * 2, 0.0f);
   apps_next_ = 0;
+  // Heads of the persistent rule queues: one per queue launch of the batch (at most three layouts
+  // per reduce group and slice), zero before the passes.
+  if (persistent) {
+    const size_t queue_cap = apps_cap * 3;
+    ensure(queue_, queue_cap * 4);
+    hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)queue_.p, queue_cap, 0.0f);
+  }
+  queue_next_ = 0;
 
   const int32_t nl = (int32_t)layers_.size();
   const int32_t last_layer = (what == 3) ? nl - 2 : nl - 1;

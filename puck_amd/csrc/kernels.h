@@ -60,6 +60,15 @@ struct FamArgs {
   int32_t mask_words, num_blocks, unary_gate;
   int32_t pitch;
   int32_t row_stride;  // floats per LDS row
+  int32_t num_chunks;  // operand chunks of the family (grid y of the plain launch)
+};
+
+// Arguments of the persistent rule kernel: the families of one span that share a layout.
+constexpr int kMaxQueueFamilies = 4;
+struct QueueArgs {
+  FamArgs fam[kMaxQueueFamilies];
+  int32_t* counter;  // queue head, zero before the launch
+  int32_t nfam;
 };
 
 struct RedArgs {
