@@ -110,7 +110,9 @@ int32_t puck_sample_sentences(const puck_grammar_t* g, uint64_t seed, int32_t n,
 /* Flags of puck_parser_create. */
 enum puck_parser_flags {
   PUCK_FLAG_ONE_ITEM_PER_LANE = 1, /* rule kernels compute one (cell, split) item per lane, not two */
-  PUCK_FLAG_PERSISTENT = 2         /* rule families of a span share one persistent work-queue launch */
+  PUCK_FLAG_PERSISTENT = 2,        /* rule families of a span share one persistent work-queue launch */
+  PUCK_FLAG_GRAPHS = 4             /* parse calls capture each batch shape's device work into a HIP
+                                      graph on its second sighting and replay it afterwards */
 };
 /* new CLParser(data, maxAllocSize) (CLParser.scala:46-48): mem_bytes bounds the device workspace,
  * as the reference's --mem option bounds WorkSpace.allocate (WorkSpace.scala:91-120). */
@@ -151,6 +153,35 @@ int32_t puck_parser_stats(const puck_parser_t* p, puck_stats_t* out);
  * the rules of the active blocks per planned item), and family id + 16 x layer index (families
  * 0-3 inside NN/NT/TN/TT, 4-7 outside-left, 8-11 outside-right, 12-15 unary). Returns the count. */
 int64_t puck_parser_launches(const puck_parser_t* p, float* ms, double* apps, int32_t* family, int64_t cap);
+
+/* ---- batched engine: many sentences of uneven length in one call ----
+ * Replaces the batch loop of CLParser.parse (WorkSpace.getBatches, WorkSpace.scala:20-41, then
+ * one batch at a time, CLParser.scala:145-206): sentences are sorted by length and cut into
+ * buckets of bucket_cells chart cells; buckets run concurrently on `streams` HIP streams (one
+ * workspace of mem_bytes / streams each), with persistent rule launches and, when `graphs` is
+ * set, each bucket shape's device work captured into a HIP graph on its second sighting and
+ * replayed after that. Results are identical to puck_parse_viterbi / puck_parse_mbr. */
+typedef struct puck_batch puck_batch_t;
+typedef struct {
+  int32_t streams;        /* 1..8 */
+  int64_t bucket_cells;   /* chart cells (sum of n(n+1)) per bucket, >= 1 */
+  int32_t persistent;     /* persistent rule launches */
+  int32_t graphs;         /* per-shape HIP graphs */
+  int32_t piece_target;   /* see puck_launch_config_t */
+  int32_t items_per_lane; /* 1 or 2 */
+} puck_batch_config_t;
+typedef struct {
+  double device_ms; /* first kernel of the first bucket to the last kernel of the last bucket */
+  int64_t buckets, graph_captures, graph_replays;
+} puck_batch_stats_t;
+int32_t puck_batch_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device, int64_t mem_bytes,
+                          const puck_batch_config_t* cfg, puck_batch_t** out);
+void puck_batch_free(puck_batch_t* b);
+int32_t puck_batch_parse_viterbi(puck_batch_t* b, const int32_t* words, const int32_t* offsets, int32_t n,
+                                 puck_tree_node_t* nodes, int64_t nodes_cap);
+int32_t puck_batch_parse_mbr(puck_batch_t* b, const int32_t* words, const int32_t* offsets, int32_t n,
+                             puck_mbr_node_t* nodes, int64_t nodes_cap);
+int32_t puck_batch_stats(puck_batch_t* b, puck_batch_stats_t* out);
 
 /* ---- serving (replaces BatchFunctionAnnotatorService, AnnotatorService.scala:29-78) ----
  * Continuous batching: up to max_inflight batches run concurrently (one HIP stream each); a free

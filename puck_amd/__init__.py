@@ -61,6 +61,17 @@ class LaunchConfig(ctypes.Structure):
 
 FLAG_ONE_ITEM_PER_LANE = 1
 FLAG_PERSISTENT = 2
+FLAG_GRAPHS = 4
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int32), ("bucket_cells", ctypes.c_int64), ("persistent", ctypes.c_int32),
+                ("graphs", ctypes.c_int32), ("piece_target", ctypes.c_int32), ("items_per_lane", ctypes.c_int32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("device_ms", ctypes.c_double), ("buckets", ctypes.c_int64), ("graph_captures", ctypes.c_int64),
+                ("graph_replays", ctypes.c_int64)]
 
 # Exported symbols and their signatures: (name, restype, argtypes).
 _P = ctypes.c_void_p
@@ -86,6 +97,14 @@ SIGNATURES = [
                                             ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
     ("puck_parser_free", None, [_P]),
     ("puck_parser_configure", ctypes.c_int32, [_P, ctypes.POINTER(LaunchConfig)]),
+    ("puck_batch_create", ctypes.c_int32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                           ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)]),
+    ("puck_batch_free", None, [_P]),
+    ("puck_batch_parse_viterbi", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(TreeNode),
+                                                  ctypes.c_int64]),
+    ("puck_batch_parse_mbr", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(MbrNode),
+                                              ctypes.c_int64]),
+    ("puck_batch_stats", ctypes.c_int32, [_P, ctypes.POINTER(BatchStats)]),
     ("puck_parse_viterbi", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(TreeNode), ctypes.c_int64]),
     ("puck_parse_mbr", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(MbrNode), ctypes.c_int64]),
     ("puck_partitions", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, _F32P]),
@@ -225,14 +244,34 @@ def synth_wsj6_cached(seed=6, cache_dir=None):
     return prefix
 
 
+def _parse_with(obj, sentences, vit_fn, mbr_fn):
+    w, wp, o, op, n = CLParser._pack(sentences)
+    lens = np.diff(o)
+    total = int(np.sum(2 * lens - 1))
+    if obj.layers[-1].semiring == VITERBI:
+        nodes = (TreeNode * max(total, 1))()
+        _check(getattr(lib(), vit_fn)(obj._h, wp, op, n, nodes, total))
+        arr = np.ctypeslib.as_array(nodes).view(np.int32).reshape(-1, 4)[:total]
+    else:
+        nodes = (MbrNode * max(total, 1))()
+        _check(getattr(lib(), mbr_fn)(obj._h, wp, op, n, nodes, total))
+        arr = np.ctypeslib.as_array(nodes).view(np.int32).reshape(-1, 5)[:total]
+    out, s = [], 0
+    for L in lens:
+        out.append(arr[s:s + 2 * L - 1].copy())
+        s += 2 * L - 1
+    return out
+
+
 class CLParser:
     """Coarse-to-fine CKY parser over grammar layers on one GPU (CLParser.scala:46-952)."""
 
-    def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2, persistent=False):
+    def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2, persistent=False, graphs=False):
         self.layers = list(layers)
         arr = (_P * len(self.layers))(*[g._h for g in self.layers])
         h = _P()
-        flags = (FLAG_ONE_ITEM_PER_LANE if items_per_lane == 1 else 0) | (FLAG_PERSISTENT if persistent else 0)
+        flags = ((FLAG_ONE_ITEM_PER_LANE if items_per_lane == 1 else 0) | (FLAG_PERSISTENT if persistent else 0)
+                 | (FLAG_GRAPHS if graphs else 0))
         _check(lib().puck_parser_create(arr, len(self.layers), device, mem_bytes, flags, ctypes.byref(h)))
         self._h = h
 
@@ -260,22 +299,7 @@ class CLParser:
 
     def parse(self, sentences):
         """Viterbi trees (list of TreeNode arrays) or MBR trees for a Real final layer."""
-        w, wp, o, op, n = self._pack(sentences)
-        lens = np.diff(o)
-        total = int(np.sum(2 * lens - 1))
-        if self.layers[-1].semiring == VITERBI:
-            nodes = (TreeNode * max(total, 1))()
-            _check(lib().puck_parse_viterbi(self._h, wp, op, n, nodes, total))
-            arr = np.ctypeslib.as_array(nodes).view(np.int32).reshape(-1, 4)[:total]
-        else:
-            nodes = (MbrNode * max(total, 1))()
-            _check(lib().puck_parse_mbr(self._h, wp, op, n, nodes, total))
-            arr = np.ctypeslib.as_array(nodes).view(np.int32).reshape(-1, 5)[:total]
-        out, s = [], 0
-        for L in lens:
-            out.append(arr[s:s + 2 * L - 1].copy())
-            s += 2 * L - 1
-        return out
+        return _parse_with(self, sentences, "puck_parse_viterbi", "puck_parse_mbr")
 
     def partitions(self, sentences):
         w, wp, o, op, n = self._pack(sentences)
@@ -330,6 +354,36 @@ class CLParser:
         lib().puck_parser_launches(self._h, ms.ctypes.data_as(_F32P), apps.ctypes.data_as(_F64P),
                                    fam.ctypes.data_as(_I32P), n)
         return ms[:n], apps[:n], fam[:n]
+
+
+class BatchParser:
+    """Batched engine (puck_batch_* in include/puck.h): sentences sorted by length into buckets of
+    `bucket_cells` chart cells that run concurrently on `streams` HIP streams, with persistent rule
+    launches and per-shape HIP graphs. Same results as CLParser.parse."""
+
+    def __init__(self, layers, device=0, mem_bytes=16 << 30, streams=3, bucket_cells=1 << 20, persistent=True,
+                 graphs=True, piece_target=512, items_per_lane=2):
+        self.layers = list(layers)
+        arr = (_P * len(self.layers))(*[g._h for g in self.layers])
+        self.config = BatchConfig(int(streams), int(bucket_cells), int(bool(persistent)), int(bool(graphs)),
+                                  int(piece_target), int(items_per_lane))
+        h = _P()
+        _check(lib().puck_batch_create(arr, len(self.layers), device, mem_bytes, ctypes.byref(self.config),
+                                       ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.puck_batch_free(self._h)
+            self._h = None
+
+    def parse(self, sentences):
+        return _parse_with(self, sentences, "puck_batch_parse_viterbi", "puck_batch_parse_mbr")
+
+    def stats(self):
+        s = BatchStats()
+        _check(lib().puck_batch_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in BatchStats._fields_}
 
 
 class Server:

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/puck.h"
+#include "batch.h"
 #include "engine.h"
 #include "server.h"
 #include "synth.h"
@@ -19,6 +20,11 @@ struct puck_grammar {
 struct puck_parser {
   std::unique_ptr<puck::Parser> p;
   mutable std::mutex mu;  // serialises calls on one handle (the reference's synchronized methods)
+};
+
+struct puck_batch {
+  std::unique_ptr<puck::BatchParser> b;
+  std::mutex mu;
 };
 
 struct puck_server {
@@ -225,7 +231,8 @@ int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers
     auto* h = new puck_parser;
     try {
       h->p.reset(new puck::Parser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, ipl));
-      h->p->persistent = (flags & PUCK_FLAG_PERSISTENT) != 0;
+      h->p->set_persistent((flags & PUCK_FLAG_PERSISTENT) != 0);
+      h->p->use_graphs = (flags & PUCK_FLAG_GRAPHS) != 0;
     } catch (...) {
       delete h;
       throw;
@@ -241,7 +248,7 @@ int32_t puck_parser_configure(puck_parser_t* p, const puck_launch_config_t* cfg)
     if (cfg->piece_target < 1 || cfg->queue_wgs_per_cu < 1 || cfg->queue_wgs_per_cu > 8)
       return set_error(PUCK_ERR_ARG, "piece_target must be >= 1 and queue_wgs_per_cu in [1, 8]");
     std::lock_guard<std::mutex> lk(p->mu);
-    p->p->persistent = cfg->persistent != 0;
+    p->p->set_persistent(cfg->persistent != 0);
     p->p->set_launch_shape(cfg->piece_target, cfg->queue_wgs_per_cu);
     return PUCK_OK;
   });
@@ -447,3 +454,79 @@ int32_t puck_server_stats(puck_server_t* h, int32_t reset, puck_server_stats_t* 
 }
 
 }  // extern "C"
+
+int32_t puck_batch_create(const puck_grammar_t* const* layers, int32_t n_layers, int32_t device, int64_t mem_bytes,
+                          const puck_batch_config_t* cfg, puck_batch_t** out) {
+  return guarded([&]() -> int32_t {
+    if (!layers || n_layers < 1 || !out || !cfg) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<std::shared_ptr<puck::Grammar>> gs;
+    for (int32_t i = 0; i < n_layers; ++i) {
+      if (!layers[i]) return set_error(PUCK_ERR_ARG, "null grammar layer");
+      gs.push_back(layers[i]->g);
+    }
+    if (cfg->items_per_lane != 1 && cfg->items_per_lane != 2) return set_error(PUCK_ERR_ARG, "items_per_lane must be 1 or 2");
+    if (cfg->piece_target < 1) return set_error(PUCK_ERR_ARG, "piece_target must be >= 1");
+    puck::BatchConfig c;
+    c.streams = cfg->streams;
+    c.bucket_cells = cfg->bucket_cells;
+    c.persistent = cfg->persistent != 0;
+    c.graphs = cfg->graphs != 0;
+    c.piece_target = cfg->piece_target;
+    c.items_per_lane = cfg->items_per_lane;
+    auto* h = new puck_batch;
+    try {
+      h->b.reset(new puck::BatchParser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, c));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return PUCK_OK;
+  });
+}
+
+void puck_batch_free(puck_batch_t* b) { delete b; }
+
+int32_t puck_batch_parse_viterbi(puck_batch_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
+                                 puck_tree_node_t* nodes, int64_t nodes_cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || n < 0 || !nodes) return set_error(PUCK_ERR_ARG, "bad argument");
+    if (nodes_needed(offsets, n) > nodes_cap) return set_error(PUCK_ERR_ARG, "nodes buffer too small");
+    std::vector<std::vector<puck::TreeNode>> trees;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->b->parse(words, offsets, n, &trees, nullptr);
+    int64_t o = 0;
+    for (auto& t : trees) {
+      std::memcpy(nodes + o, t.data(), t.size() * sizeof(puck::TreeNode));
+      o += (int64_t)t.size();
+    }
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_batch_parse_mbr(puck_batch_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
+                             puck_mbr_node_t* nodes, int64_t nodes_cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || n < 0 || !nodes) return set_error(PUCK_ERR_ARG, "bad argument");
+    if (nodes_needed(offsets, n) > nodes_cap) return set_error(PUCK_ERR_ARG, "nodes buffer too small");
+    std::vector<std::vector<puck::MbrNode>> trees;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->b->parse(words, offsets, n, nullptr, &trees);
+    int64_t o = 0;
+    for (auto& t : trees) {
+      std::memcpy(nodes + o, t.data(), t.size() * sizeof(puck::MbrNode));
+      o += (int64_t)t.size();
+    }
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_batch_stats(puck_batch_t* h, puck_batch_stats_t* out) {
+  if (!h || !out) return set_error(PUCK_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(h->mu);
+  out->device_ms = h->b->stats.device_ms;
+  out->buckets = h->b->stats.buckets;
+  out->graph_captures = h->b->stats.graph_captures;
+  out->graph_replays = h->b->stats.graph_replays;
+  return PUCK_OK;
+}

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -138,6 +139,9 @@ struct PassStats {
   std::vector<float> launch_ms;      // per rule-kernel launch duration
   std::vector<double> launch_apps;   // per rule-kernel launch rule applications
   std::vector<int32_t> launch_family;
+  int64_t graph_captures = 0;  // batches whose device work was captured into a graph
+  int64_t graph_replays = 0;   // batches replayed from a captured graph
+  double span_lo_ms = 1e300, span_hi_ms = -1e300;  // device interval of the batches from ref_event
 };
 
 class Parser {
@@ -171,6 +175,9 @@ class Parser {
                     bool outside, Charts* out);
 
   PassStats stats;
+  // When set, each batch's device interval is also measured from this event (recorded on any
+  // stream of the same device before the call): stats.span_lo_ms / span_hi_ms.
+  hipEvent_t ref_event = nullptr;
   const Layer& layer(int i) const { return *layers_[i]; }
   int32_t num_layers() const { return (int32_t)layers_.size(); }
   // Per-launch timing (HIP events around every rule launch); off for serving.
@@ -184,12 +191,34 @@ class Parser {
   void set_launch_shape(int32_t piece_target, int32_t queue_wgs_per_cu) {
     piece_target_ = piece_target;
     queue_wgs_per_cu_ = queue_wgs_per_cu;
+    drop_graphs();
   }
+  void set_persistent(bool on) {
+    persistent = on;
+    drop_graphs();
+  }
+  // Drops every captured graph (their launches no longer match the configuration).
+  void drop_graphs();
   int32_t piece_target_ = 512;
   int32_t queue_wgs_per_cu_ = 1;
+  // Graph mode of parse(): the device work of a batch is captured into a HIP graph the second
+  // time a batch of the same sentence lengths (in order) is seen, and replayed after that.
+  bool use_graphs = false;
 
  private:
   struct Batch;
+  struct CaptureAbort {};
+  struct GraphEntry {
+    int32_t seen = 0;       // eager runs of this shape
+    bool failed = false;    // capture abandoned: stay eager
+    hipGraphExec_t exec = nullptr;
+    int64_t generation = 0; // workspace generation the graph's addresses belong to
+  };
+  static constexpr size_t kMaxGraphs = 64;
+  std::map<std::vector<int32_t>, GraphEntry> graphs_;
+  bool capturing_ = false;
+  int64_t ws_generation_ = 0;  // bumped whenever a workspace buffer is reallocated
+  GraphEntry* graph_entry(const std::vector<int32_t>& lengths);
   std::vector<std::unique_ptr<Layer>> layers_;
   int32_t device_;
   int64_t mem_bytes_;
@@ -205,7 +234,7 @@ class Parser {
   struct Phase { hipEvent_t a, b; double* acc; };
   std::vector<Phase> phases_;
   hipEvent_t phase_start_ = nullptr;
-  hipEvent_t dev_start_ = nullptr;
+  hipEvent_t dev_start_ = nullptr, dev_end_ = nullptr;
   bool dev_marked_ = true;
   void mark_device_end();
   hipEvent_t next_event();
@@ -224,6 +253,17 @@ class Parser {
                  std::vector<std::vector<MbrNode>>* mbr, float* partitions, Charts* charts,
                  int32_t chart_layer, std::vector<uint32_t>* masks_out);
   std::vector<std::pair<int32_t, int32_t>> make_batches(const int32_t* offsets, int32_t n);
+  void parse_batch(Batch& b, int32_t first, std::vector<std::vector<TreeNode>>* viterbi,
+                   std::vector<std::vector<MbrNode>>* mbr);
+  void parse_device(Batch& b);
+  void finish_parse(Batch& b, int32_t first, std::vector<std::vector<TreeNode>>* viterbi,
+                    std::vector<std::vector<MbrNode>>* mbr);
+  void finish_batch(Batch& b, int32_t what, float* partitions, Charts* charts, int32_t chart_layer,
+                    std::vector<uint32_t>* masks_out);
+  void reset_counters(Batch& b);
+  void launch_masks_and_scales(Batch& b, int32_t li, int32_t mask_words);
+  void launch_viterbi(Batch& b, int32_t li);
+  void launch_mbr(Batch& b, int32_t li);
   void inside_pass(Batch& b, int32_t li, bool use_masks);
   void outside_pass(Batch& b, int32_t li, bool use_masks);
   bool family_args(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks, int32_t k0,

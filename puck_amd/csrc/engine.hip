@@ -1468,6 +1468,7 @@ struct Parser::Batch {
   int32_t* d_prefix = nullptr;
   std::vector<int32_t> h_prefix;
   std::vector<double> partition_scales;
+  size_t apps_cap = 0;   // plan counters (doubles) of the batch; queue heads use 3x as many ints
   int32_t mask_in = 0;   // which half of masks_ holds the current input masks
   bool has_masks = false;
   bool has_scales = false;
@@ -1486,7 +1487,9 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
       throw std::runtime_error("all layers must share the coarse symbol set");
     layers_.emplace_back(new Layer(*layers[i], i + 1 == layers.size(), i > 0, items_per_lane));
   }
-  HIP_CHECK(hipStreamCreate(&stream_));
+  // Non-blocking: work of other parsers' streams (and of the null stream) never orders against
+  // this one, so parsers on one device run concurrently.
+  HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreate(&ev0_));
   HIP_CHECK(hipEventCreate(&ev1_));
   // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
@@ -1514,15 +1517,38 @@ Parser::~Parser() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
                  &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_})
     if (b->p) (void)hipFree(b->p);
+  drop_graphs();
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+Parser::GraphEntry* Parser::graph_entry(const std::vector<int32_t>& lengths) {
+  auto it = graphs_.find(lengths);
+  if (it == graphs_.end()) {
+    if (graphs_.size() >= kMaxGraphs) {
+      for (auto& kv : graphs_)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+      graphs_.clear();
+    }
+    it = graphs_.emplace(lengths, GraphEntry{}).first;
+  }
+  GraphEntry& e = it->second;
+  if (e.exec != nullptr && e.generation != ws_generation_) {
+    // A workspace buffer moved since the capture: capture again.
+    HIP_CHECK(hipGraphExecDestroy(e.exec));
+    e.exec = nullptr;
+  }
+  return &e;
+}
+
 void Parser::ensure(Buf& b, size_t bytes) {
   bytes = std::max<size_t>(bytes, 256);
   if (b.bytes >= bytes) return;
+  // A graph being captured cannot take new memory; graphs captured before hold the old address.
+  if (capturing_) throw CaptureAbort();
+  ++ws_generation_;
   if (b.p) HIP_CHECK(hipFree(b.p));
   b.p = nullptr;
   b.bytes = 0;
@@ -1557,10 +1583,12 @@ std::vector<std::pair<int32_t, int32_t>> Parser::make_batches(const int32_t* off
 
 // Phase timing with pooled events, resolved once per batch in flush_timing (no host syncs).
 void Parser::timed_begin() {
+  if (capturing_) return;
   phase_start_ = next_event();
   HIP_CHECK(hipEventRecord(phase_start_, stream_));
 }
 void Parser::timed_end(double* acc) {
+  if (capturing_) return;
   hipEvent_t e = next_event();
   HIP_CHECK(hipEventRecord(e, stream_));
   phases_.push_back(Phase{phase_start_, e, acc});
@@ -1765,13 +1793,22 @@ void Parser::mark_device_end() {
   hipEvent_t e = next_event();
   HIP_CHECK(hipEventRecord(e, stream_));
   phases_.push_back(Phase{dev_start_, e, &stats.device_ms});
+  dev_end_ = e;
+}
+
+void Parser::drop_graphs() {
+  for (auto& kv : graphs_)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  graphs_.clear();
 }
 
 void Parser::flush_timing() {
   HIP_CHECK(hipStreamSynchronize(stream_));
   std::vector<double> plan_apps(apps_next_, 0.0);
-  if (apps_next_ > 0)
-    HIP_CHECK(hipMemcpy(plan_apps.data(), apps_.p, (size_t)apps_next_ * 8, hipMemcpyDeviceToHost));
+  if (apps_next_ > 0 && !pending_.empty()) {
+    HIP_CHECK(hipMemcpyAsync(plan_apps.data(), apps_.p, (size_t)apps_next_ * 8, hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+  }
   for (const TimedLaunch& t : pending_) {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
@@ -1791,6 +1828,14 @@ void Parser::flush_timing() {
     HIP_CHECK(hipEventElapsedTime(&ms, ph.a, ph.b));
     *ph.acc += ms;
   }
+  if (ref_event != nullptr && dev_start_ != nullptr && dev_end_ != nullptr) {
+    float lo = 0, hi = 0;
+    HIP_CHECK(hipEventElapsedTime(&lo, ref_event, dev_start_));
+    HIP_CHECK(hipEventElapsedTime(&hi, ref_event, dev_end_));
+    stats.span_lo_ms = std::min(stats.span_lo_ms, (double)lo);
+    stats.span_hi_ms = std::max(stats.span_hi_ms, (double)hi);
+  }
+  dev_end_ = nullptr;
   pending_.clear();
   phases_.clear();
   ev_next_ = 0;
@@ -2122,18 +2167,17 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   // Plan counters: one double per planned (layer, span, family group member) of the batch.
   // (Four reduce groups per span and layer at most: inside binary and unary, outside binary and
   // unary; room for eight partial-buffer slices each.)
-  const size_t apps_cap = (size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies * 8;
-  ensure(apps_, apps_cap * 8);
-  hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)apps_.p, apps_cap * 2, 0.0f);
-  apps_next_ = 0;
+  B.apps_cap = (size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies * 8;
+  ensure(apps_, B.apps_cap * 8);
   // Heads of the persistent rule queues: one per queue launch of the batch (at most three layouts
-  // per reduce group and slice), zero before the passes.
-  if (persistent) {
-    const size_t queue_cap = apps_cap * 3;
-    ensure(queue_, queue_cap * 4);
-    hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)queue_.p, queue_cap, 0.0f);
+  // per reduce group and slice).
+  if (persistent) ensure(queue_, B.apps_cap * 3 * 4);
+
+  if (what == 0) {
+    parse_batch(B, first, viterbi, mbr);
+    return;
   }
-  queue_next_ = 0;
+  reset_counters(B);
 
   const int32_t nl = (int32_t)layers_.size();
   const int32_t last_layer = (what == 3) ? nl - 2 : nl - 1;
@@ -2150,25 +2194,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
     if (!is_last) {
       timed_begin();
       outside_pass(B, li, use_masks);
-      // computePruningMasks: masks at threshold -9 (CLParser.scala:223-229).
-      const int32_t mask_out = B.has_masks ? 1 - B.mask_in : 0;
-      uint32_t* mo = (uint32_t*)masks_.p + (size_t)mask_out * B.num_cells * mask_words;
-      hipLaunchKernelGGL(k_masks, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_, mo,
-                         (const float*)inside_.p, (const float*)outside_.p, B.d_cell_offsets,
-                         B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n, g.num_syms,
-                         g.cell_size, g.root, -9.0f, mask_words);
-      HIP_CHECK(hipGetLastError());
-      if (layers_[li + 1]->g.semiring == kReal) {
-        hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
-                           (float*)in_scale_.p, (const float*)inside_.p, g.cell_size, B.num_cells);
-        hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
-                           (float*)out_scale_.p, (const float*)outside_.p, g.cell_size, B.num_cells);
-        B.has_scales = true;
-      } else {
-        B.has_scales = false;
-      }
-      B.mask_in = mask_out;
-      B.has_masks = true;
+      launch_masks_and_scales(B, li, mask_words);
       timed_end(&stats.total_ms);
       if (what == 4 && li == chart_layer) break;
       continue;
@@ -2209,37 +2235,190 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
         outside_pass(B, li, use_masks);
         timed_end(&stats.total_ms);
       }
-      if (what == 0 || what == 2) {
+      if (what == 2) {
         ensure(trees_, (size_t)B.tree_offsets.back() * sizeof(TreeNode));
         timed_begin();
-        hipLaunchKernelGGL(k_viterbi, dim3(B.n), dim3(64), 0, stream_, (TreeNode*)trees_.p,
-                           B.d_tree_offsets, (const float*)inside_.p, B.d_cell_offsets, B.d_lengths,
-                           B.n, g.cell_size, g.root, g.num_nonterms(), L.d_dec_offsets,
-                           L.d_dec_scores, L.d_dec_left, L.d_dec_right);
-        HIP_CHECK(hipGetLastError());
+        launch_viterbi(B, li);
         timed_end(&stats.other_ms);
-        if (what == 0) {
-          std::vector<TreeNode> all(B.tree_offsets.back());
-          mark_device_end();
-          HIP_CHECK(hipMemcpyAsync(all.data(), trees_.p, all.size() * sizeof(TreeNode), hipMemcpyDeviceToHost, stream_));
-          HIP_CHECK(hipStreamSynchronize(stream_));
-          for (int32_t i = 0; i < B.n; ++i)
-            (*viterbi)[first + i].assign(all.begin() + B.tree_offsets[i], all.begin() + B.tree_offsets[i + 1]);
-        }
       }
     } else {
       timed_begin();
       outside_pass(B, li, use_masks);
       timed_end(&stats.total_ms);
-      if (what == 0 || what == 2) {
+      if (what == 2) {
         ensure(mbr_, (size_t)B.num_cells * sizeof(MbrCell));
-        hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
-                           (MbrCell*)mbr_.p, (const float*)inside_.p, (const float*)outside_.p,
-                           B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n,
-                           g.num_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size());
-        HIP_CHECK(hipGetLastError());
+        launch_mbr(B, li);
       }
-      if (what == 0) {
+    }
+  }
+  finish_batch(B, what, partitions, charts, chart_layer, masks_out);
+}
+
+// computePruningMasks: masks at threshold -9 (CLParser.scala:223-229) into the half of masks_ not
+// holding this layer's input masks, and the next layer's scaling constants when it is Real
+// (CLParser.scala:236-245).
+void Parser::launch_masks_and_scales(Batch& B, int32_t li, int32_t mask_words) {
+  const Layer& L = *layers_[li];
+  const Grammar& g = L.g;
+  const int32_t mask_out = B.has_masks ? 1 - B.mask_in : 0;
+  uint32_t* mo = (uint32_t*)masks_.p + (size_t)mask_out * B.num_cells * mask_words;
+  hipLaunchKernelGGL(k_masks, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_, mo,
+                     (const float*)inside_.p, (const float*)outside_.p, B.d_cell_offsets,
+                     B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n, g.num_syms,
+                     g.cell_size, g.root, -9.0f, mask_words);
+  HIP_CHECK(hipGetLastError());
+  if (layers_[li + 1]->g.semiring == kReal) {
+    hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+                       (float*)in_scale_.p, (const float*)inside_.p, g.cell_size, B.num_cells);
+    hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+                       (float*)out_scale_.p, (const float*)outside_.p, g.cell_size, B.num_cells);
+    HIP_CHECK(hipGetLastError());
+    B.has_scales = true;
+  } else {
+    B.has_scales = false;
+  }
+  B.mask_in = mask_out;
+  B.has_masks = true;
+}
+
+void Parser::launch_viterbi(Batch& B, int32_t li) {
+  const Layer& L = *layers_[li];
+  const Grammar& g = L.g;
+  hipLaunchKernelGGL(k_viterbi, dim3(B.n), dim3(64), 0, stream_, (TreeNode*)trees_.p,
+                     B.d_tree_offsets, (const float*)inside_.p, B.d_cell_offsets, B.d_lengths,
+                     B.n, g.cell_size, g.root, g.num_nonterms(), L.d_dec_offsets,
+                     L.d_dec_scores, L.d_dec_left, L.d_dec_right);
+  HIP_CHECK(hipGetLastError());
+}
+
+void Parser::launch_mbr(Batch& B, int32_t li) {
+  const Layer& L = *layers_[li];
+  const Grammar& g = L.g;
+  hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+                     (MbrCell*)mbr_.p, (const float*)inside_.p, (const float*)outside_.p,
+                     B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n,
+                     g.num_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size());
+  HIP_CHECK(hipGetLastError());
+}
+
+void Parser::reset_counters(Batch& B) {
+  hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)apps_.p, B.apps_cap * 2, 0.0f);
+  if (persistent)
+    hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)queue_.p, B.apps_cap * 3, 0.0f);
+  apps_next_ = 0;
+  queue_next_ = 0;
+}
+
+// Device work of CLParser.parse on one batch: every layer's passes, the masks between layers and
+// the final decode kernel. Enqueues only (no host waits), so it can be captured into a graph.
+void Parser::parse_device(Batch& B) {
+  reset_counters(B);
+  B.has_masks = false;
+  B.has_scales = false;
+  B.mask_in = 0;
+  const int32_t nl = (int32_t)layers_.size();
+  const int32_t mask_words = layers_[0]->g.mask_words;
+  for (int32_t li = 0; li < nl; ++li) {
+    const Grammar& g = layers_[li]->g;
+    timed_begin();
+    inside_pass(B, li, B.has_masks);
+    timed_end(&stats.total_ms);
+    if (li + 1 < nl) {
+      timed_begin();
+      outside_pass(B, li, B.has_masks);
+      launch_masks_and_scales(B, li, mask_words);
+      timed_end(&stats.total_ms);
+      continue;
+    }
+    if (g.semiring == kViterbi) {
+      timed_begin();
+      launch_viterbi(B, li);
+      timed_end(&stats.other_ms);
+    } else {
+      timed_begin();
+      outside_pass(B, li, B.has_masks);
+      timed_end(&stats.total_ms);
+      launch_mbr(B, li);
+    }
+  }
+}
+
+// CLParser.parse on one batch: the device work (replayed from a graph when this batch's shape
+// was captured before), then the result download and the host side of the decoders.
+void Parser::parse_batch(Batch& B, int32_t first, std::vector<std::vector<TreeNode>>* viterbi,
+                         std::vector<std::vector<MbrNode>>* mbr) {
+  const bool viterbi_final = layers_.back()->g.semiring == kViterbi;
+  if (viterbi_final) ensure(trees_, (size_t)B.tree_offsets.back() * sizeof(TreeNode));
+  else ensure(mbr_, (size_t)B.num_cells * sizeof(MbrCell));
+  GraphEntry* ge = use_graphs ? graph_entry(B.lengths) : nullptr;
+  if (ge != nullptr && ge->exec != nullptr) {
+    HIP_CHECK(hipGraphLaunch(ge->exec, stream_));
+    ++stats.graph_replays;
+  } else if (ge != nullptr && ge->seen > 0 && !ge->failed) {
+    // Second sighting of this shape: capture the device work and launch the graph. Workspace
+    // buffers were sized by the first (eager) run; a capture that would need to grow one is
+    // abandoned and the batch runs eagerly.
+    const PassStats saved = stats;
+    const bool saved_timing = timing;
+    timing = false;
+    capturing_ = true;
+    HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    bool aborted = false;
+    try {
+      parse_device(B);
+    } catch (const CaptureAbort&) {
+      aborted = true;
+    } catch (...) {
+      hipGraph_t gdrop = nullptr;
+      (void)hipStreamEndCapture(stream_, &gdrop);
+      if (gdrop) (void)hipGraphDestroy(gdrop);
+      capturing_ = false;
+      timing = saved_timing;
+      throw;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream_, &graph);
+    capturing_ = false;
+    timing = saved_timing;
+    if (!aborted && ec == hipSuccess && graph != nullptr &&
+        hipGraphInstantiate(&ge->exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      ge->generation = ws_generation_;
+      HIP_CHECK(hipGraphDestroy(graph));
+      HIP_CHECK(hipGraphLaunch(ge->exec, stream_));
+      ++stats.graph_captures;
+    } else {
+      if (graph != nullptr) (void)hipGraphDestroy(graph);
+      ge->exec = nullptr;
+      ge->failed = true;
+      (void)hipGetLastError();
+      stats = saved;
+      parse_device(B);
+    }
+  } else {
+    parse_device(B);
+    if (ge != nullptr) ge->seen += 1;
+  }
+  // Host state the device work leaves behind (also when it was replayed).
+  B.has_masks = layers_.size() > 1;
+  B.has_scales = layers_.size() > 1 && !viterbi_final;
+  finish_parse(B, first, viterbi, mbr);
+  flush_timing();
+}
+
+void Parser::finish_parse(Batch& B, int32_t first, std::vector<std::vector<TreeNode>>* viterbi,
+                          std::vector<std::vector<MbrNode>>* mbr) {
+  if (layers_.back()->g.semiring == kViterbi) {
+    std::vector<TreeNode> all(B.tree_offsets.back());
+    mark_device_end();
+    HIP_CHECK(hipMemcpyAsync(all.data(), trees_.p, all.size() * sizeof(TreeNode), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int32_t i = 0; i < B.n; ++i)
+      (*viterbi)[first + i].assign(all.begin() + B.tree_offsets[i], all.begin() + B.tree_offsets[i + 1]);
+    return;
+  }
+  {
+    {
+      {
         std::vector<MbrCell> dec(B.num_cells);
         std::vector<float> isc(B.num_cells), osc(B.num_cells);
         mark_device_end();
@@ -2311,7 +2490,14 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
       }
     }
   }
+}
 
+// Downloads of the non-parse calls: debug charts and coarse masks; resolves the timing events.
+void Parser::finish_batch(Batch& B, int32_t what, float* partitions, Charts* charts, int32_t chart_layer,
+                          std::vector<uint32_t>* masks_out) {
+  (void)what;
+  (void)partitions;
+  const int32_t mask_words = layers_[0]->g.mask_words;
   mark_device_end();
   if (charts != nullptr) {
     const Grammar& g = layers_[chart_layer]->g;
