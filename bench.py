@@ -4,11 +4,18 @@ Workload (the headline configuration): exhaustive Viterbi parsing with the synth
 grammar (1,084 refined states = 502 nonterminals + 582 tags, 1,725,570 binary and 125,619 unary
 rules; the real wsj_6 rule files are not in the reference checkout) of 2,000 seeded PCFG-sampled
 sentences, lengths 1-40 from a discretized log-normal (mean ~23). One step = one parse of the
-2,000 sentences: tag scores, inside pass (one launch per span and rule family), Viterbi decode.
+2,000 sentences: tag scores, inside pass, Viterbi decode.
 
-Prints one JSON line. `value` is sentences/s from device time (HIP events from the first kernel
-after the word upload to the tree download); the PCIe-inclusive host-API rate is reported beside
-it as `e2e_value`.
+Two forms run on the same sentences and must return identical trees:
+  plain    one launch per span and rule family, one batch (the reference's schedule)
+  batched  the batched engine: length buckets on concurrent HIP streams, one persistent
+           work-queue launch per span for all rule families, per-shape HIP graphs
+`value` is the batched engine's sentences/s from device time (HIP events from its first kernel to
+its last; inputs resident); `plain` and `batched_vs_plain` stand beside it. The roofline is that of
+the persistent rule kernel (k_rules_q) over the same work, from per-launch HIP events of a timed
+persistent parse on the engine stream. Side workloads: coarse-to-fine Viterbi and MBR (5,000
+sentences), mixed lengths to 80 (20,000 sentences), each plain vs batched, and Poisson serving
+with p50/p99 latency. Prints one JSON line.
 """
 import argparse
 import ctypes
@@ -62,11 +69,11 @@ def cpu_baseline(puck, prefix, words, offsets, sample, threads):
                        f"OpenMP over sentences, {dt:.1f} s; grammar load {load_s:.1f} s excluded)"), trees, tree_off
 
 
-PROFILE = os.path.join(ROOT, "profiles", "k_rules_wsj6_exhaustive_viterbi.md")
+PROFILE = os.path.join(ROOT, "profiles", "k_rules_q_wsj6_exhaustive_viterbi.md")
 
 
 def profiled_traffic():
-    """HBM bytes per k_rules launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    """HBM bytes per k_rules_q launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
     this workload (copy-calibrated), or None when the summary is absent."""
     try:
         with open(PROFILE) as f:
@@ -87,6 +94,43 @@ def timed_parses(parser, words, offsets, steps):
         parser.parse((words, offsets))
         dev += parser.stats()["device_ms"]
     return dev / steps, parser.launches()
+
+
+def timed_batched(bp, words, offsets, steps):
+    """Eager run, graph capture run, then `steps` replays; device ms per parse and the trees."""
+    trees = None
+    for _ in range(2):
+        trees = bp.parse((words, offsets))
+    dev = 0.0
+    for _ in range(steps):
+        trees = bp.parse((words, offsets))
+        dev += bp.stats()["device_ms"]
+    return dev / steps, trees
+
+
+def same_trees(a, b):
+    return len(a) == len(b) and all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def plain_vs_batched(puck, layers, words, offsets, steps, cfg, label):
+    """Plain and batched forms on one workload: rates, ratio, tree identity, fine-layer roofline."""
+    lens = np.diff(offsets)
+    p = puck.CLParser(layers, device=0, mem_bytes=64 << 30)
+    ms_p, launches = timed_parses(p, words, offsets, steps)
+    ref = p.parse((words, offsets))
+    del p
+    bp = puck.BatchParser(layers, mem_bytes=96 << 30, **cfg)
+    ms_b, trees = timed_batched(bp, words, offsets, steps)
+    st = bp.stats()
+    del bp
+    out = {"workload": label, "sentences": int(len(lens)), "mean_length": round(float(lens.mean()), 2),
+           "value": round(len(lens) / (ms_b / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms_b, 2),
+           "plain": round(len(lens) / (ms_p / 1e3), 1), "plain_ms_per_parse": round(ms_p, 2),
+           "batched_vs_plain": round(ms_p / ms_b, 3), "trees_identical": same_trees(trees, ref),
+           "buckets": int(st["buckets"]), "graph_replays": int(st["graph_replays"]),
+           "roofline_plain": fine_roofline(launches)}
+    print(f"[bench] {label}: batched {out['value']} vs plain {out['plain']} sentences/s", file=sys.stderr, flush=True)
+    return out
 
 
 def fine_roofline(launches):
@@ -126,37 +170,23 @@ def serve(puck, layers, pool, rate, seconds, seed):
             "mean_batch": round(st["mean_batch"], 1)}
 
 
-def side_workloads(puck, g6v, prefix, args):
+def side_workloads(puck, g6v, prefix, args, cfg):
     """Coarse-to-fine, mixed-length and serving workloads, measured beside the headline."""
     out = {}
     g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
     words, offsets = g6v.sample(seed=3, n=args.c2f_sentences, mode=0, min_len=1, max_len=40)
-    lens = np.diff(offsets)
-    p = puck.CLParser([g1, g6v], device=0, mem_bytes=64 << 30)
-    ms, launches = timed_parses(p, words, offsets, 2)
-    out["c2f_viterbi"] = {
-        "workload": f"wsj_1 -> synthetic wsj_6-shape Viterbi, masks at -9, {len(lens)} PCFG sentences 1-40",
-        "value": round(len(lens) / (ms / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms, 2),
-        "mean_length": round(float(lens.mean()), 2), "roofline": fine_roofline(launches)}
-    print(f"[bench] c2f viterbi {out['c2f_viterbi']['value']} sentences/s", file=sys.stderr, flush=True)
+    out["c2f_viterbi"] = plain_vs_batched(
+        puck, [g1, g6v], words, offsets, 2, cfg,
+        f"wsj_1 -> synthetic wsj_6-shape Viterbi, masks at -9, {args.c2f_sentences} PCFG sentences 1-40")
     g6r = puck.Grammar.load_text(prefix, puck.REAL)
-    pr = puck.CLParser([g1, g6r], device=0, mem_bytes=64 << 30)
-    ms, launches = timed_parses(pr, words, offsets, 2)
-    out["c2f_mbr"] = {
-        "workload": f"wsj_1 -> synthetic wsj_6-shape Real inside/outside + MBR, {len(lens)} PCFG sentences 1-40",
-        "value": round(len(lens) / (ms / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms, 2),
-        "roofline": fine_roofline(launches)}
-    print(f"[bench] c2f mbr {out['c2f_mbr']['value']} sentences/s", file=sys.stderr, flush=True)
-    del pr, g6r
+    out["c2f_mbr"] = plain_vs_batched(
+        puck, [g1, g6r], words, offsets, 2, cfg,
+        f"wsj_1 -> synthetic wsj_6-shape Real inside/outside + MBR, {args.c2f_sentences} PCFG sentences 1-40")
+    del g6r
     mw, mo = g6v.sample(seed=4, n=args.mixed_sentences, mode=2, min_len=1, max_len=80)
-    ml = np.diff(mo)
-    ms, launches = timed_parses(p, mw, mo, 1)
-    out["mixed_lengths"] = {
-        "workload": f"c2f Viterbi, {len(ml)} sentences, 70% log-normal 1-40 + 30% uniform 41-80",
-        "value": round(len(ml) / (ms / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms, 2),
-        "mean_length": round(float(ml.mean()), 2), "roofline": fine_roofline(launches)}
-    print(f"[bench] mixed lengths {out['mixed_lengths']['value']} sentences/s", file=sys.stderr, flush=True)
-    del p
+    out["mixed_lengths"] = plain_vs_batched(
+        puck, [g1, g6v], mw, mo, 1, cfg,
+        f"c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
     pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
     pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
     out["serving"] = []
@@ -165,6 +195,18 @@ def side_workloads(puck, g6v, prefix, args):
         out["serving"].append(r)
         print(f"[bench] serving {r}", file=sys.stderr, flush=True)
     return out
+
+
+TUNED = os.path.join(ROOT, "puck_amd", "tuning", "batched_wsj6.json")
+
+
+def batched_config(args):
+    """The autotuned configuration (puck_amd/autotune.py) when one is committed, else defaults."""
+    from puck_amd import autotune
+    cfg = dict(autotune.DEFAULTS)
+    if os.path.exists(TUNED) and not args.no_tuned:
+        cfg.update(autotune.load(TUNED)["best"])
+    return cfg
 
 
 def main():
@@ -180,7 +222,8 @@ def main():
     ap.add_argument("--check", type=int, default=8, help="sentences whose trees are checked against the oracle")
     ap.add_argument("--side", type=int, default=1, help="also run the c2f, mixed-length and serving workloads")
     ap.add_argument("--c2f-sentences", type=int, default=5000)
-    ap.add_argument("--mixed-sentences", type=int, default=2000)
+    ap.add_argument("--mixed-sentences", type=int, default=20000)
+    ap.add_argument("--no-tuned", action="store_true", help="use the default batched configuration")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="seconds of Poisson arrivals per rate")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (counter calibration)")
@@ -207,32 +250,52 @@ def main():
     words, offsets = g.sample(seed=args.seed, n=args.sentences, mode=1, min_len=1, max_len=40)
     setup_s = time.time() - t0
     lens = np.diff(offsets)
-    parser = puck.CLParser([g], device=0, mem_bytes=64 << 30, items_per_lane=args.ipl)
+    cfg = batched_config(args)
     print(f"[bench] setup {setup_s:.1f}s; {len(lens)} sentences, mean length {lens.mean():.1f}, "
-          f"max {lens.max()}", file=sys.stderr, flush=True)
+          f"max {lens.max()}; batched config {cfg}", file=sys.stderr, flush=True)
 
-    for i in range(args.warmup):
-        parser.parse((words, offsets))
-        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
-    hip_sync()
-    dev_ms, wall, kern_ms, kern_apps, launches = 0.0, 0.0, 0.0, 0.0, 0
-    trees = None
+    # Plain form (one launch per span and family) on the same sentences.
+    plain = puck.CLParser([g], device=0, mem_bytes=64 << 30, items_per_lane=args.ipl)
+    ms_plain, _ = timed_parses(plain, words, offsets, args.steps)
+    ref_trees = plain.parse((words, offsets))
+    del plain
+    print(f"[bench] plain {ms_plain:.1f} ms per parse", file=sys.stderr, flush=True)
+
+    # Roofline of the persistent rule kernel: per-launch HIP events on the engine stream over timed
+    # persistent parses of the same batch the batched engine runs.
+    pers = puck.CLParser([g], device=0, mem_bytes=64 << 30, items_per_lane=cfg["items_per_lane"], persistent=True)
+    pers.configure(True, piece_target=cfg["piece_target"])
+    pers.parse((words, offsets))
+    kern_ms, kern_apps, launches = 0.0, 0.0, 0
     for i in range(args.steps):
-        t = time.time()
-        trees = parser.parse((words, offsets))
-        hip_sync()
-        wall += time.time() - t
-        st = parser.stats()
-        dev_ms += st["device_ms"]
-        ms, apps, fam = parser.launches()
-        binary = fam < 4  # inside binary families: the dominant kernel
+        pers.parse((words, offsets))
+        ms, apps, fam = pers.launches()
+        binary = fam == 0  # the inside binary families' queue launches: the dominant kernel
         kern_ms += float(ms[binary].sum())
         kern_apps += float(apps[binary].sum())
         launches += int(binary.sum())
-        print(f"[bench] step {i}: device {st['device_ms']:.1f} ms, wall {time.time() - t:.2f} s",
-              file=sys.stderr, flush=True)
+    del pers
+
+    # Batched engine: the timed steps.
+    bp = puck.BatchParser([g], mem_bytes=96 << 30, **cfg)
+    for i in range(max(args.warmup, 2)):
+        bp.parse((words, offsets))
+        print(f"[bench] batched warmup {i} done", file=sys.stderr, flush=True)
+    hip_sync()
+    dev_ms, wall = 0.0, 0.0
+    trees = None
+    for i in range(args.steps):
+        t = time.time()
+        trees = bp.parse((words, offsets))
+        hip_sync()
+        wall += time.time() - t
+        st = bp.stats()
+        dev_ms += st["device_ms"]
+        print(f"[bench] step {i}: device {st['device_ms']:.1f} ms, wall {time.time() - t:.2f} s, "
+              f"buckets {st['buckets']}, graph replays {st['graph_replays']}", file=sys.stderr, flush=True)
+    del bp
     n = len(lens) * args.steps
-    default_workload = (args.sentences, args.seed, args.ipl) == (2000, 2, 2)
+    default_workload = (args.sentences, args.seed) == (2000, 2)
     value = n / (dev_ms / 1e3)
     achieved = 3.0 * kern_apps / (kern_ms / 1e3)
     line = {
@@ -241,7 +304,7 @@ def main():
         "unit": "sentences/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": max(args.warmup, 2),
         "ms_per_step": round(dev_ms / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -251,24 +314,29 @@ def main():
         "config": {"workload": "exhaustive Viterbi parse (tags + inside + decode), synthetic wsj_6-shape grammar, "
                                f"{len(lens)} sentences, lengths 1-40 log-normal",
                    "sentences": int(len(lens)), "mean_length": round(float(lens.mean()), 2),
-                   "mode": "plain (one launch per span and rule family)", "items_per_lane": args.ipl},
+                   "mode": "batched (length buckets, concurrent streams, persistent rule launches, HIP graphs)",
+                   "batched": cfg},
+        "plain": {"value": round(len(lens) / (ms_plain / 1e3), 2), "ms_per_step": round(ms_plain, 3),
+                  "mode": "plain (one launch per span and rule family)"},
+        "batched_vs_plain": round(ms_plain / (dev_ms / args.steps), 3),
+        "trees_identical_to_plain": same_trees(trees, ref_trees),
         "e2e_value": round(n / wall, 2),
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(VALU_PEAK / 1e12, 2),
                      "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK, 4),
                      "traffic": profiled_traffic() if default_workload else None,
-                     "kernel": "k_rules (inside binary families)", "launches": launches,
-                     "avg_launch_ms": round(kern_ms / max(launches, 1), 4),
+                     "kernel": "k_rules_q (inside binary families, one persistent launch per span)",
+                     "launches": launches, "avg_launch_ms": round(kern_ms / max(launches, 1), 4),
                      "ops_per_launch": round(3.0 * kern_apps / max(launches, 1), 1)},
     }
     # Parity spot check against the oracle, and the CPU baseline on a bounded sample.
     if args.cpu_sample > 0:
-        base, ref_trees, tree_off = cpu_baseline(puck, prefix, words, offsets, args.cpu_sample, args.cpu_threads)
+        base, ref_o, tree_off = cpu_baseline(puck, prefix, words, offsets, args.cpu_sample, args.cpu_threads)
         line["cpu_baseline"] = base
         k = min(args.check, args.cpu_sample)
-        ok = all(np.array_equal(trees[i], ref_trees[tree_off[i]:tree_off[i + 1]]) for i in range(k))
+        ok = all(np.array_equal(trees[i], ref_o[tree_off[i]:tree_off[i + 1]]) for i in range(k))
         line["trees_match_oracle"] = f"{k}/{k}" if ok else "MISMATCH"
     if args.side:
-        line.update(side_workloads(puck, g, prefix, args))
+        line.update(side_workloads(puck, g, prefix, args, cfg))
     print(json.dumps(line))
 
 

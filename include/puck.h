@@ -185,7 +185,8 @@ int32_t puck_batch_stats(puck_batch_t* b, puck_batch_stats_t* out);
 
 /* ---- serving (replaces BatchFunctionAnnotatorService, AnnotatorService.scala:29-78) ----
  * Continuous batching: up to max_inflight batches run concurrently (one HIP stream each); a free
- * slot admits queued requests in arrival order while their chart cells fit its memory budget. */
+ * slot admits queued requests in arrival order while their chart cells fit its memory budget.
+ * flags: PUCK_FLAG_ONE_ITEM_PER_LANE and PUCK_FLAG_PERSISTENT as for puck_parser_create. */
 typedef struct puck_server puck_server_t;
 typedef struct {
   int64_t completed, batches;

@@ -390,11 +390,12 @@ class Server:
     """Continuous-batching parse server (replaces BatchFunctionAnnotatorService,
     AnnotatorService.scala:29-78): submit sentences from any thread, collect results by ticket."""
 
-    def __init__(self, layers, device=0, mem_bytes=16 << 30, max_inflight=2, max_batch=20000, items_per_lane=2):
+    def __init__(self, layers, device=0, mem_bytes=16 << 30, max_inflight=2, max_batch=20000, items_per_lane=2,
+                 persistent=True):
         self.layers = list(layers)
         arr = (_P * len(self.layers))(*[g._h for g in self.layers])
         h = _P()
-        flags = 1 if items_per_lane == 1 else 0
+        flags = (FLAG_ONE_ITEM_PER_LANE if items_per_lane == 1 else 0) | (FLAG_PERSISTENT if persistent else 0)
         _check(lib().puck_server_create(arr, len(self.layers), device, mem_bytes, max_inflight, max_batch, flags,
                                         ctypes.byref(h)))
         self._h = h

@@ -1,0 +1,103 @@
+"""Autotuner of the batched engine's launch shape and chart packing.
+
+The reference fixes its device work shapes when it generates kernels: rule segments of a coarse
+parent block split into 8 sub-segments per kernel (CoarseParentSymbolGrammarClusterer.java:132-150),
+work groups of 96 work-items (JavaFriendlyGenRuleMultiply.scala:62-63) and batches closed by
+workspace size (WorkSpace.scala:20-41). The knobs of this engine that play those roles are:
+
+  piece_target    how finely each wave's rule stream is cut into work units (1-16 pieces per
+                  stream: the rules one workgroup runs per unit)
+  items_per_lane  items per lane of the rule kernels: 64 or 128 (cell, split) items per
+                  1,024-thread workgroup
+  persistent      one work-queue launch per span for all rule families, or one launch per family
+  bucket_cells    chart cells per length bucket (chart packing)
+  streams         buckets in flight on separate HIP streams
+
+tune() measures device time of the batched engine on a sample of the caller's workload, one knob
+at a time from the defaults (coordinate descent, two rounds), checks that every configuration
+returns the same trees as the plain parser, and returns the best configuration with the table of
+measurements. save()/load() keep the result as JSON.
+"""
+import json
+import time
+
+import numpy as np
+
+DEFAULTS = dict(streams=3, bucket_cells=1 << 22, persistent=True, graphs=True, piece_target=512, items_per_lane=2)
+SPACE = dict(
+    streams=[1, 2, 3, 4],
+    bucket_cells=[1 << 20, 1 << 21, 1 << 22, 1 << 23],
+    piece_target=[256, 512, 1024],
+    items_per_lane=[1, 2],
+    persistent=[False, True],
+)
+
+
+def _measure(puck, layers, sents, cfg, mem_bytes, reps):
+    bp = puck.BatchParser(layers, mem_bytes=mem_bytes, **cfg)
+    trees = None
+    # First call runs eagerly, the second captures the bucket graphs, later calls replay them.
+    for _ in range(2):
+        trees = bp.parse(sents)
+    best = float("inf")
+    for _ in range(reps):
+        trees = bp.parse(sents)
+        best = min(best, bp.stats()["device_ms"])
+    st = bp.stats()
+    del bp
+    return best, trees, st
+
+
+def tune(puck, layers, sentences, mem_bytes=96 << 30, rounds=2, reps=2, log=None):
+    """Returns {"best": cfg, "best_ms": t, "plain_ms": t0, "table": [...]} for the workload sample
+    `sentences` ((words, offsets) or a list of word-id arrays)."""
+    plain = puck.CLParser(layers, mem_bytes=mem_bytes)
+    ref = plain.parse(sentences)
+    plain.parse(sentences)
+    plain_ms = plain.stats()["device_ms"]
+    del plain
+    table = []
+    seen = {}
+
+    def run(cfg):
+        key = tuple(sorted(cfg.items()))
+        if key in seen:
+            return seen[key]
+        t0 = time.time()
+        ms, trees, st = _measure(puck, layers, sentences, cfg, mem_bytes, reps)
+        same = len(trees) == len(ref) and all(np.array_equal(a, b) for a, b in zip(trees, ref))
+        row = dict(cfg, device_ms=round(ms, 3), buckets=int(st["buckets"]), same_trees=bool(same),
+                   wall_s=round(time.time() - t0, 2))
+        table.append(row)
+        if log:
+            log(row)
+        if not same:
+            raise RuntimeError(f"configuration {cfg} changed the parse results")
+        seen[key] = ms
+        return ms
+
+    best = dict(DEFAULTS)
+    best_ms = run(best)
+    for _ in range(rounds):
+        improved = False
+        for knob, values in SPACE.items():
+            for v in values:
+                if best[knob] == v:
+                    continue
+                cand = dict(best, **{knob: v})
+                ms = run(cand)
+                if ms < best_ms * 0.99:
+                    best, best_ms, improved = cand, ms, True
+        if not improved:
+            break
+    return {"best": best, "best_ms": best_ms, "plain_ms": plain_ms, "table": table}
+
+
+def save(path, result, workload):
+    with open(path, "w") as f:
+        json.dump(dict(result, workload=workload), f, indent=1)
+
+
+def load(path):
+    with open(path) as f:
+        return json.load(f)

@@ -389,7 +389,8 @@ int32_t puck_server_create(const puck_grammar_t* const* layers, int32_t n_layers
     h->mbr = gs.back()->semiring == puck::kReal;
     try {
       h->s.reset(new puck::Server(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, max_inflight,
-                                  max_batch > 0 ? max_batch : 20000, (flags & 1) ? 1 : 2));
+                                  max_batch > 0 ? max_batch : 20000, (flags & PUCK_FLAG_ONE_ITEM_PER_LANE) ? 1 : 2,
+                                  (flags & PUCK_FLAG_PERSISTENT) != 0));
     } catch (...) {
       delete h;
       throw;

@@ -7,7 +7,7 @@
 namespace puck {
 
 Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
-               int32_t max_inflight, int32_t max_batch, int32_t items_per_lane)
+               int32_t max_inflight, int32_t max_batch, int32_t items_per_lane, bool persistent)
     : max_batch_(std::max(1, max_batch)) {
   if (max_inflight < 1) throw std::runtime_error("max_inflight must be >= 1");
   mbr_final_ = layers.back()->semiring == kReal;
@@ -19,6 +19,7 @@ Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   for (int32_t i = 0; i < max_inflight; ++i) {
     parsers_.emplace_back(new Parser(layers, device, per_slot, items_per_lane));
     parsers_.back()->timing = false;
+    parsers_.back()->set_persistent(persistent);
   }
   for (int32_t i = 0; i < max_inflight; ++i) threads_.emplace_back(&Server::worker, this, i);
 }

@@ -32,7 +32,7 @@ struct ServerStats {
 class Server {
  public:
   Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
-         int32_t max_inflight, int32_t max_batch, int32_t items_per_lane);
+         int32_t max_inflight, int32_t max_batch, int32_t items_per_lane, bool persistent);
   ~Server();
 
   int64_t submit(const int32_t* words, int32_t len);
