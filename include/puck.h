@@ -78,6 +78,13 @@ int32_t puck_abi_version(void);
 /* ---- grammars (CLParserData.make over SimpleGrammar.parseBerkeleyText(prefix, -12, ...)) ---- */
 int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out);
 void puck_grammar_free(puck_grammar_t* g);
+/* CLParserData.writeSequence / readSequence (CLParser.scala:1249-1317) with ZipUtil
+ * (ZipUtil.scala:13-98): the compiled layers in one zip ("grammar.grz") with the reference's entry
+ * names (parserNames, parser-i/grammar, parser-i/structure, parser-i/semiring) and binary table
+ * payloads of this library. load returns the number of layers and writes that many handles when
+ * cap is large enough (otherwise it only returns the count). */
+int32_t puck_grammar_save_grz(const puck_grammar_t* const* layers, int32_t n_layers, const char* path);
+int32_t puck_grammar_load_grz(const char* path, puck_grammar_t** out, int32_t cap);
 int32_t puck_grammar_info(const puck_grammar_t* g, puck_grammar_info_t* out);
 /* Word id of `word` in the grammar lexicon, -1 if unknown. */
 int32_t puck_grammar_word_id(const puck_grammar_t* g, const char* word);
@@ -89,6 +96,9 @@ int32_t puck_grammar_rules(const puck_grammar_t* g, int32_t family, int32_t* idx
 /* Symbol table: coarse id and substate of nonterminal (terminal=0) or terminal (terminal=1) ids. */
 int32_t puck_grammar_symbols(const puck_grammar_t* g, int32_t terminal, int32_t* coarse, int32_t* sub,
                              int32_t cap);
+/* Name of coarse symbol `coarse` (.numstates line order), NUL-terminated into buf[cap]; returns its
+ * length. */
+int32_t puck_grammar_coarse_name(const puck_grammar_t* g, int32_t coarse, char* buf, int32_t cap);
 /* CoarseParent blocks: which = 0 inside, 1 outside-left, 2 outside-right; returns #blocks and
  * writes block_of_coarse[num_coarse]. */
 int32_t puck_grammar_blocks(const puck_grammar_t* g, int32_t which, int32_t family,

@@ -84,10 +84,13 @@ SIGNATURES = [
     ("puck_abi_version", ctypes.c_int32, []),
     ("puck_grammar_load_text", ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(_P)]),
     ("puck_grammar_free", None, [_P]),
+    ("puck_grammar_save_grz", ctypes.c_int32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_char_p]),
+    ("puck_grammar_load_grz", ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.c_int32]),
     ("puck_grammar_info", ctypes.c_int32, [_P, ctypes.POINTER(GrammarInfo)]),
     ("puck_grammar_word_id", ctypes.c_int32, [_P, ctypes.c_char_p]),
     ("puck_grammar_rules", ctypes.c_int32, [_P, ctypes.c_int32, _I32P, _F32P, _F32P, ctypes.c_int32]),
     ("puck_grammar_symbols", ctypes.c_int32, [_P, ctypes.c_int32, _I32P, _I32P, ctypes.c_int32]),
+    ("puck_grammar_coarse_name", ctypes.c_int32, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]),
     ("puck_grammar_blocks", ctypes.c_int32, [_P, ctypes.c_int32, ctypes.c_int32, _I32P]),
     ("puck_grammar_lexicon", ctypes.c_int32, [_P, ctypes.c_int32, _I32P, _F64P, ctypes.c_int32]),
     ("puck_synth_wsj6", ctypes.c_int32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64]),
@@ -178,6 +181,19 @@ class Grammar:
         _check(lib().puck_grammar_load_text(prefix.encode(), semiring, ctypes.byref(h)))
         return cls(h, prefix, semiring)
 
+    @classmethod
+    def load_grz(cls, path):
+        """All layers of a grammar.grz container (CLParserData.readSequence), coarse first."""
+        n = _check(lib().puck_grammar_load_grz(path.encode(), None, 0))
+        hs = (_P * n)()
+        _check(lib().puck_grammar_load_grz(path.encode(), hs, n))
+        out = []
+        for h in hs:
+            info = GrammarInfo()
+            _check(lib().puck_grammar_info(h, ctypes.byref(info)))
+            out.append(cls(_P(h), f"{path}#{len(out)}", info.semiring))
+        return out
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.puck_grammar_free(self._h)
@@ -203,6 +219,15 @@ class Grammar:
         _check(lib().puck_grammar_symbols(self._h, int(terminal), c.ctypes.data_as(_I32P), s.ctypes.data_as(_I32P), n))
         return c[:n], s[:n]
 
+    def coarse_names(self):
+        """Coarse symbol names in .numstates order."""
+        out = []
+        buf = ctypes.create_string_buffer(256)
+        for c in range(self.info.num_coarse):
+            _check(lib().puck_grammar_coarse_name(self._h, c, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
     def blocks(self, which, family):
         boc = np.zeros(self.info.num_coarse, np.int32)
         nb = _check(lib().puck_grammar_blocks(self._h, which, family, boc.ctypes.data_as(_I32P)))
@@ -223,6 +248,12 @@ class Grammar:
         _check(lib().puck_sample_sentences(self._h, seed, n, mode, min_len, max_len, words.ctypes.data_as(_I32P),
                                            cap, offsets.ctypes.data_as(_I32P)))
         return words[:offsets[-1]].copy(), offsets
+
+
+def save_grz(path, layers):
+    """Writes the compiled layers (coarse first) into one grammar.grz (CLParserData.writeSequence)."""
+    arr = (_P * len(layers))(*[g._h for g in layers])
+    _check(lib().puck_grammar_save_grz(arr, len(layers), path.encode()))
 
 
 def synth_wsj6(out_prefix, seed=6, data_dir=DATA_DIR):

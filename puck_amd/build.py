@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libpuck.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PUCK_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["grammar.cpp", "synth.cpp", "server.cpp", "batch.cpp", "capi.cpp"]
+HOST_SOURCES = ["grammar.cpp", "grz.cpp", "synth.cpp", "server.cpp", "batch.cpp", "capi.cpp"]
 DEVICE_SOURCES = ["engine.hip"]
 
 COMMON_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result"]

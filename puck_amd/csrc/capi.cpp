@@ -1,4 +1,5 @@
 // extern "C" boundary over the grammar compiler, input generators and the CKY engine.
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -10,6 +11,7 @@
 #include "../../include/puck.h"
 #include "batch.h"
 #include "engine.h"
+#include "grz.h"
 #include "server.h"
 #include "synth.h"
 
@@ -93,6 +95,34 @@ int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_gramma
 
 void puck_grammar_free(puck_grammar_t* g) { delete g; }
 
+int32_t puck_grammar_save_grz(const puck_grammar_t* const* layers, int32_t n_layers, const char* path) {
+  return guarded([&]() -> int32_t {
+    if (!layers || n_layers < 1 || !path) return set_error(PUCK_ERR_ARG, "bad argument");
+    std::vector<const puck::Grammar*> gs;
+    for (int32_t i = 0; i < n_layers; ++i) {
+      if (!layers[i]) return set_error(PUCK_ERR_ARG, "null grammar layer");
+      gs.push_back(layers[i]->g.get());
+    }
+    puck::save_grz(path, gs);
+    return PUCK_OK;
+  });
+}
+
+int32_t puck_grammar_load_grz(const char* path, puck_grammar_t** out, int32_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!path) return set_error(PUCK_ERR_ARG, "null argument");
+    auto gs = puck::load_grz(path);
+    if ((int32_t)gs.size() > cap) return (int32_t)gs.size();
+    if (!out) return set_error(PUCK_ERR_ARG, "null output array");
+    for (size_t i = 0; i < gs.size(); ++i) {
+      auto* h = new puck_grammar;
+      h->g = gs[i];
+      out[i] = h;
+    }
+    return (int32_t)gs.size();
+  });
+}
+
 int32_t puck_grammar_info(const puck_grammar_t* h, puck_grammar_info_t* out) {
   return guarded([&]() -> int32_t {
     if (!h || !out) return set_error(PUCK_ERR_ARG, "null argument");
@@ -166,6 +196,19 @@ int32_t puck_grammar_symbols(const puck_grammar_t* h, int32_t terminal, int32_t*
       if (sub) sub[i] = v[i].sub;
     }
     return (int32_t)v.size();
+  });
+}
+
+int32_t puck_grammar_coarse_name(const puck_grammar_t* h, int32_t coarse, char* buf, int32_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || coarse < 0 || coarse >= (int32_t)h->g->coarse_names.size()) return set_error(PUCK_ERR_ARG, "bad argument");
+    const std::string& s = h->g->coarse_names[coarse];
+    if (buf && cap > 0) {
+      const size_t n = std::min<size_t>(s.size(), (size_t)cap - 1);
+      std::memcpy(buf, s.data(), n);
+      buf[n] = 0;
+    }
+    return (int32_t)s.size();
   });
 }
 

@@ -63,3 +63,30 @@ def test_batched_exhaustive_single_layer(puck, wsj1, mixed):
     bp = puck.BatchParser([wsj1], mem_bytes=3 << 30, streams=4, bucket_cells=3000, persistent=False)
     for k in range(3):
         assert _same(bp.parse(mixed), ref), f"call {k}"
+
+
+def test_grz_layers_parse_identically(puck, wsj0, wsj1, mixed, tmp_path):
+    path = str(tmp_path / "grammar.grz")
+    puck.save_grz(path, [wsj0, wsj1])
+    loaded = puck.Grammar.load_grz(path)
+    a = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30).parse(mixed)
+    b = puck.CLParser(loaded, mem_bytes=2 << 30).parse(mixed)
+    assert _same(a, b)
+
+
+def test_cli_compile_and_parse(puck, tmp_path):
+    from puck_amd import cli
+    grz = str(tmp_path / "g.grz")
+    cli.main(["compile", "--textGrammarPrefix", puck.grammar_path("wsj_0") + ":" + puck.grammar_path("wsj_1"),
+              "--grammar", grz])
+    src = tmp_path / "in.txt"
+    long_line = " ".join(["the"] * 60)
+    src.write_text("The company said it expects higher profits .\n\nInvestors were flurbing again .\n"
+                   + long_line + "\nShe sold shares .\n")
+    cli.main(["parse", "--grammar", grz, "--mem", "2g", str(src)])
+    out = (tmp_path / "in.txt.parsed").read_text().splitlines()
+    assert len(out) == 4
+    assert out[2] == "(())"  # maxLength 50
+    for k in (0, 1, 3):
+        assert out[k].startswith("(") and out[k].count("(") == out[k].count(")") and out[k] != "(())"
+    assert "company" in out[0] and "flurbing" in out[1]

@@ -1,0 +1,77 @@
+"""grammar.grz container (CPU): a zip with the reference's entry names whose binary payloads
+load back into identical compiled tables; corrupted or truncated files are rejected."""
+import zipfile
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def grz(tmp_path_factory, puck, wsj0, wsj1):
+    path = str(tmp_path_factory.mktemp("grz") / "grammar.grz")
+    puck.save_grz(path, [wsj0, wsj1])
+    return path
+
+
+def test_zip_layout_and_entry_names(grz, puck):
+    with zipfile.ZipFile(grz) as z:
+        names = z.namelist()
+        assert names[0] == "parserNames"
+        for i in range(2):
+            for e in ("grammar", "structure", "semiring"):
+                assert f"parser-{i}/{e}" in names
+        assert z.testzip() is None  # CRCs agree
+        lines = z.read("parserNames").decode().split()
+        assert lines == [puck.grammar_path("wsj_0"), puck.grammar_path("wsj_1")]
+        assert z.read("parser-1/semiring") == b"Viterbi"
+
+
+def test_tables_round_trip(grz, puck, wsj0, wsj1):
+    back = puck.Grammar.load_grz(grz)
+    assert len(back) == 2
+    for a, b in zip([wsj0, wsj1], back):
+        for f in ("num_nonterms", "num_terms", "num_coarse", "cell_size", "mask_words", "root",
+                  "num_unary_n", "num_unary_t", "num_words", "semiring", "rules_dropped"):
+            assert getattr(a.info, f) == getattr(b.info, f), f
+        assert list(a.info.num_binary) == list(b.info.num_binary)
+        assert list(a.info.num_blocks) == list(b.info.num_blocks)
+        for fam in range(6):
+            for x, y in zip(a.rules(fam), b.rules(fam)):
+                assert np.array_equal(x, y)
+        for t in (0, 1):
+            for x, y in zip(a.symbols(t), b.symbols(t)):
+                assert np.array_equal(x, y)
+        for which in range(3):
+            for fam in range(4):
+                na, ba = a.blocks(which, fam)
+                nb, bb = b.blocks(which, fam)
+                assert na == nb and np.array_equal(ba, bb)
+        for wid in (0, 7, 1234, a.info.num_words - 1):
+            for x, y in zip(a.lexicon(wid), b.lexicon(wid)):
+                assert np.array_equal(x, y)
+        assert a.word_id("the") == b.word_id("the")
+
+
+def test_real_semiring_layer(tmp_path, puck, wsj0, wsj1_real):
+    path = str(tmp_path / "real.grz")
+    puck.save_grz(path, [wsj0, wsj1_real])
+    back = puck.Grammar.load_grz(path)
+    assert back[0].info.semiring == puck.VITERBI and back[1].info.semiring == puck.REAL
+    _, _, qa = wsj1_real.rules(0)
+    _, _, qb = back[1].rules(0)
+    assert np.array_equal(qa, qb)
+
+
+def test_corrupted_and_truncated_files_are_rejected(tmp_path, grz, puck):
+    data = bytearray(open(grz, "rb").read())
+    bad = tmp_path / "bad.grz"
+    data[200] ^= 0xFF  # inside the first payload
+    bad.write_bytes(bytes(data))
+    with pytest.raises(puck.PuckError, match="CRC|bad grz"):
+        puck.Grammar.load_grz(str(bad))
+    short = tmp_path / "short.grz"
+    short.write_bytes(open(grz, "rb").read()[:1000])
+    with pytest.raises(puck.PuckError, match="bad grz"):
+        puck.Grammar.load_grz(str(short))
+    with pytest.raises(puck.PuckError, match="cannot open"):
+        puck.Grammar.load_grz(str(tmp_path / "missing.grz"))
