@@ -268,14 +268,16 @@ class Parser {
   void outside_pass(Batch& b, int32_t li, bool use_masks);
   bool family_args(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks, int32_t k0,
                    int32_t kc, float* partial, const int32_t* plan_counts, const int32_t* plan_items,
-                   FamArgs* out, dim3* grid, size_t* lds);
+                   const uint32_t* plan_act, FamArgs* out, dim3* grid, size_t* lds);
   void launch_family(Batch& b, int32_t li, int32_t fid, int32_t span, bool use_masks,
                      int32_t slice_cell0, int32_t slice_cells, float* partial_base,
-                     const int32_t* plan_counts, const int32_t* plan_items, int32_t apps_slot);
+                     const int32_t* plan_counts, const int32_t* plan_items, const uint32_t* plan_act,
+                     int32_t apps_slot);
   void launch_queue(Batch& b, int32_t li, const std::vector<int32_t>& fids, int32_t span, bool use_masks,
                     int32_t k0, int32_t kc, const std::vector<float*>& partials,
                     const std::vector<const int32_t*>& plan_counts,
-                    const std::vector<const int32_t*>& plan_items, const std::vector<int32_t>& apps_slots);
+                    const std::vector<const int32_t*>& plan_items,
+                    const std::vector<const uint32_t*>& plan_acts, const std::vector<int32_t>& apps_slots);
   void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks);
   void timed_begin();
   void timed_end(double* acc);

@@ -177,6 +177,15 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       const int ct = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
       if (a.masks == nullptr) {
         gate[0] = gate[1] = gate[2] = gate[3] = 0xffffffffu;
+      } else if (a.plan_items != nullptr) {
+        // Planned items have allowed children and a target with an allowed span; the plan kept
+        // the target cell's active blocks (a unary item's one block is the unary family).
+        if (UNARY) {
+          gate[0] = 1u;
+        } else {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) gate[w] = a.plan_act[(size_t)w * a.num_cells + k];
+        }
       } else if (UNARY) {
         // UnaryUpdateManager.doUpdates gates on the top mask of the cell (CLParser.scala:747).
         const int ctop = cell_abs(a.cell_offsets, sent, n, 1, b, e);
@@ -212,28 +221,32 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
 
   // Stage operand rows: consecutive threads read consecutive symbols of one cell; eight loads are
   // in flight per thread before the LDS stores.
+  // Element idx of the tile's rows is (item idx / rows, row idx % rows); a thread's elements are
+  // kBlockThreads apart, so it steps its (item, row) pair by a fixed quotient and remainder (one
+  // division per thread instead of one per element).
   auto stage = [&](const float* ch, const int32_t* cells, int lo, int rows, float* dst) {
     const int total = rows * ITEMS;
+    const int step_it = kBlockThreads / rows, step_row = kBlockThreads - step_it * rows;
+    int it = tid / rows, row = tid - it * rows;
     for (int base = tid; base < total; base += 8 * kBlockThreads) {
       float v[8];
+      int at[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int idx = base + k * kBlockThreads;
         v[k] = zero;
-        if (idx < total) {
-          const int it = idx / rows;
-          const int row = idx - it * rows;
-          if (tile0 + it < num_items) v[k] = ch[(size_t)cells[it] * a.cell_size + lo + row];
+        at[k] = row * a.row_stride + it;
+        if (idx < total && tile0 + it < num_items) v[k] = ch[(size_t)cells[it] * a.cell_size + lo + row];
+        row += step_row;
+        it += step_it;
+        if (row >= rows) {
+          row -= rows;
+          ++it;
         }
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int idx = base + k * kBlockThreads;
-        if (idx < total) {
-          const int it = idx / rows;
-          dst[(idx - it * rows) * a.row_stride + it] = v[k];
-        }
-      }
+      for (int k = 0; k < 8; ++k)
+        if (base + k * kBlockThreads < total) dst[at[k]] = v[k];
     }
   };
   stage(a.chA, s_cell_a, a_lo, rows_a, lds_a);
@@ -1600,7 +1613,7 @@ bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
 
 bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks, int32_t k0,
                          int32_t kc, float* partial, const int32_t* plan_counts, const int32_t* plan_items,
-                         FamArgs* out, dim3* grid, size_t* lds) {
+                         const uint32_t* plan_act, FamArgs* out, dim3* grid, size_t* lds) {
   Layer& L = *layers_[li];
   const DevFamily& F = L.fam[fid];
   const Grammar& g = L.g;
@@ -1621,6 +1634,7 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.num_cells = kc;
   int64_t nitems;
   a.plan_items = plan_items;
+  a.plan_act = plan_act;
   a.slice_cell0 = 0;  // planned item codes are slice-relative
   if (plan_items != nullptr) {
     // Planned slice: its item count is on the device; the grid covers the unpruned count.
@@ -1679,13 +1693,14 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
 
 void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks,
                            int32_t k0, int32_t kc, float* partial, const int32_t* plan_counts,
-                           const int32_t* plan_items, int32_t apps_slot) {
+                           const int32_t* plan_items, const uint32_t* plan_act, int32_t apps_slot) {
   Layer& L = *layers_[li];
   const DevFamily& F = L.fam[fid];
   FamArgs a;
   dim3 grid;
   size_t lds;
-  if (!family_args(B, li, fid, span, use_masks, k0, kc, partial, plan_counts, plan_items, &a, &grid, &lds)) return;
+  if (!family_args(B, li, fid, span, use_masks, k0, kc, partial, plan_counts, plan_items, plan_act, &a, &grid, &lds))
+    return;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (timing) {
     t0 = next_event();
@@ -1720,7 +1735,8 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
 void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids, int32_t span,
                           bool use_masks, int32_t k0, int32_t kc, const std::vector<float*>& partials,
                           const std::vector<const int32_t*>& plan_counts,
-                          const std::vector<const int32_t*>& plan_items, const std::vector<int32_t>& apps_slots) {
+                          const std::vector<const int32_t*>& plan_items,
+                          const std::vector<const uint32_t*>& plan_acts, const std::vector<int32_t>& apps_slots) {
   Layer& L = *layers_[li];
   QueueArgs q{};
   size_t lds = 0;
@@ -1737,8 +1753,8 @@ void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids
     dim3 grid;
     size_t l = 0;
     FamArgs a;
-    if (!family_args(B, li, fids[i], span, use_masks, k0, kc, partials[i], plan_counts[i], plan_items[i], &a,
-                     &grid, &l))
+    if (!family_args(B, li, fids[i], span, use_masks, k0, kc, partials[i], plan_counts[i], plan_items[i],
+                     plan_acts[i], &a, &grid, &l))
       continue;
     if (q.nfam == kMaxQueueFamilies) throw std::logic_error("too many families in one queue launch");
     q.fam[q.nfam++] = a;
@@ -1983,7 +1999,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       ra.pitch[fi] = pitch;
       if (!persistent)
         launch_family(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? pfs[fi].pre : nullptr,
-                      use_masks ? pfs[fi].items : nullptr, apps_slot[fi]);
+                      use_masks ? pfs[fi].items : nullptr, use_masks ? pfs[fi].act : nullptr, apps_slot[fi]);
       off += (size_t)pitch * F.num_slots;
     }
     if (persistent) {
@@ -1992,6 +2008,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
         std::vector<int32_t> fids, slots;
         std::vector<float*> parts;
         std::vector<const int32_t*> pcs, pis;
+        std::vector<const uint32_t*> pas;
         for (size_t fi = 0; fi < nfam; ++fi) {
           const DevFamily& F = L.fam[R.families[fi]];
           const int lay = F.unary ? 0 : (F.skip_layout ? 1 : 2);
@@ -2000,9 +2017,10 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
           parts.push_back(fam_partial[fi]);
           pcs.push_back(use_masks ? pfs[fi].pre : nullptr);
           pis.push_back(use_masks ? pfs[fi].items : nullptr);
+          pas.push_back(use_masks ? pfs[fi].act : nullptr);
           slots.push_back(apps_slot[fi]);
         }
-        if (!fids.empty()) launch_queue(B, li, fids, span, use_masks, k0, kc, parts, pcs, pis, slots);
+        if (!fids.empty()) launch_queue(B, li, fids, span, use_masks, k0, kc, parts, pcs, pis, pas, slots);
       }
     }
     float* inside = (float*)inside_.p;

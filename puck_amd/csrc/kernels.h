@@ -44,6 +44,7 @@ struct FamArgs {
   // = uniform count. With plan_items, the prefix is the mask plan's and items are listed there.
   const int32_t* prefix;
   const int32_t* plan_items;  // planned items of the span: (cell << 8) | split, null = all items
+  const uint32_t* plan_act;   // planned binary families: active-block bits, 4 rows of num_cells words
   const int32_t* chunk_range;
   // Per (chunk, wave): num_block_ranges + 1 group starts, the wave's first slot id, then the
   // split tables: for P = 1, 2, 4, ..., P + 1 (group, first slot) pairs.
