@@ -101,3 +101,31 @@ def save(path, result, workload):
 def load(path):
     with open(path) as f:
         return json.load(f)
+
+
+def main(argv=None):
+    """python -m puck_amd.autotune --out FILE: tunes the batched engine on a seeded sample of the
+    mixed-length coarse-to-fine workload (wsj_1 -> synthetic wsj_6 shape, 70% log-normal 1-40 +
+    30% uniform 41-80) and writes the result as JSON."""
+    import argparse
+    import sys
+    import puck_amd as puck
+    ap = argparse.ArgumentParser(prog="puck_amd.autotune")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--sentences", type=int, default=4000)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--rounds", type=int, default=2)
+    args = ap.parse_args(argv)
+    g6 = puck.Grammar.load_text(puck.synth_wsj6_cached(seed=6), puck.VITERBI)
+    g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
+    sents = g6.sample(seed=args.seed, n=args.sentences, mode=2, min_len=1, max_len=80)
+    res = tune(puck, [g1, g6], sents, rounds=args.rounds,
+               log=lambda row: print(f"[autotune] {row}", file=sys.stderr, flush=True))
+    workload = (f"wsj_1 -> synthetic wsj_6-shape c2f Viterbi, {args.sentences} sentences (seed {args.seed}), "
+                "70% log-normal 1-40 + 30% uniform 41-80")
+    save(args.out, res, workload)
+    print(json.dumps({"best": res["best"], "best_ms": res["best_ms"], "plain_ms": res["plain_ms"]}))
+
+
+if __name__ == "__main__":
+    main()
