@@ -557,6 +557,11 @@ __global__ void k_plan(PlanArgs p) {
       }
     }
     f.keys[k] = key;
+    // A family without key blocks keeps the slice order (no sort pass runs for it).
+    if (f.key_blocks[0] < 0) {
+      f.perm[k] = k;
+      f.posof[k] = k;
+    }
   }
 }
 
@@ -570,6 +575,7 @@ __global__ __launch_bounds__(512) void k_plan_sort(PlanArgs p) {
   __shared__ int32_t s_a[D * T];
   __shared__ int32_t s_sum[T];
   const PlanFam& f = p.fam[blockIdx.y];
+  if (f.key_blocks[0] < 0) return;  // order already written by k_plan
   const int t = threadIdx.x;
   const int per = (p.num_cells + T - 1) / T;
   const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
@@ -927,39 +933,61 @@ struct Best {
   int i;
 };
 
+__device__ inline Best better(Best a, Best b) {
+  return (b.s > a.s || (b.s == a.s && b.i > a.i)) ? b : a;
+}
+
 __device__ inline Best wave_best(Best b) {
   for (int o = 32; o > 0; o >>= 1) {
-    float os = __shfl_xor(b.s, o);
-    int oi = __shfl_xor(b.i, o);
-    if (os > b.s || (os == b.s && oi > b.i)) {
-      b.s = os;
-      b.i = oi;
-    }
+    Best x{__shfl_xor(b.s, o), __shfl_xor(b.i, o)};
+    b = better(b, x);
   }
   return b;
 }
 
-__global__ void k_viterbi(TreeNode* tree_out, const int32_t* tree_offsets, const float* inside,
-                          const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
-                          int32_t cell_size, int32_t root, int32_t nn, const int32_t* offsets,
-                          const float* scores, const int32_t* lefts, const int32_t* rights) {
+constexpr int kViterbiThreads = 256;
+
+// Best of the workgroup's candidates: highest score, ties to the larger candidate index (the
+// candidate the reference's sequential scan with >= keeps). Every thread returns the result.
+__device__ inline Best block_best(Best b, Best* s_best) {
+  b = wave_best(b);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_best[w] = b;
+  __syncthreads();
+  Best r = s_best[0];
+  for (int k = 1; k < kViterbiThreads / 64; ++k) r = better(r, s_best[k]);
+  __syncthreads();
+  return r;
+}
+
+// Viterbi decode (CLViterbi.scala:364-608): one workgroup per sentence walks the tree top-down in
+// pre-order; at each node its threads scan strided subsets of the candidates (unary rules of the
+// node's top symbol, then NT, TN, TT and NN rules of its bottom symbol over the splits) and the
+// workgroup keeps the best, ties to the larger candidate index -- the candidate the reference's
+// sequential scan with >= selects.
+__global__ __launch_bounds__(kViterbiThreads) void k_viterbi(
+    TreeNode* tree_out, const int32_t* tree_offsets, const float* inside, const int32_t* cell_offsets,
+    const int32_t* lengths, int32_t num_sent, int32_t cell_size, int32_t root, int32_t nn,
+    const int32_t* offsets, const float* scores, const int32_t* lefts, const int32_t* rights) {
+  constexpr int VT = kViterbiThreads;
   __shared__ int32_t s_top[2 * kMaxLength], s_bot[2 * kMaxLength], s_width[2 * kMaxLength];
   __shared__ float s_score[2 * kMaxLength];
+  __shared__ Best s_best[VT / 64];
   const int sent = blockIdx.x;
   if (sent >= num_sent) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   const int n = lengths[sent];
   const float* ibot = inside + (size_t)cell_offsets[sent] * cell_size;
   const float* itop = ibot + (size_t)tri(n) * cell_size;
   const int nodes = 2 * n - 1;
-  for (int p = lane; p < nodes; p += 64) {
+  for (int p = tid; p < nodes; p += VT) {
     s_top[p] = 0;
     s_bot[p] = 0;
     s_width[p] = 0;
     s_score[p] = 0.0f;
   }
   __syncthreads();
-  if (lane == 0) {
+  if (tid == 0) {
     s_top[0] = root;
     s_width[0] = n;
   }
@@ -970,28 +998,29 @@ __global__ void k_viterbi(TreeNode* tree_out, const int32_t* tree_offsets, const
     const int width = s_width[p];
     const int end = begin + width;
     int bot = s_bot[p];
-    if (s_top[p] != -1) {
+    const int top = s_top[p];
+    if (top != -1) {
       // BestUnary (CLViterbi.scala:383-425): init -30000, sym 0.
       const int g = (width == 1) ? g_ut : g_u;
-      const int r0 = offsets[g + s_top[p]], r1 = offsets[g + s_top[p] + 1];
+      const int r0 = offsets[g + top], r1 = offsets[g + top + 1];
       const float* cell = ibot + (size_t)chart_index(begin, end, n) * cell_size;
       Best b{-30000.0f, -1};
-      for (int r = r0 + lane; r < r1; r += 64) {
+      for (int r = r0 + tid; r < r1; r += VT) {
         const float sc = scores[r] + cell[lefts[r]];
         if (sc >= b.s) {
           b.s = sc;
           b.i = r;
         }
       }
-      b = wave_best(b);
+      b = block_best(b, s_best);
       bot = b.i >= 0 ? lefts[b.i] : 0;
-      if (lane == 0) {
+      if (tid == 0) {
         s_bot[p] = bot;
         s_score[p] = b.s;
       }
     }
     if (width == 1) {
-      if (lane == 0) s_score[p] = ibot[(size_t)cell_size * begin + bot];
+      if (tid == 0) s_score[p] = ibot[(size_t)cell_size * begin + bot];
       begin += 1;
     } else {
       const int nt0 = offsets[g_nt + bot], nt1 = offsets[g_nt + bot + 1];
@@ -1002,14 +1031,14 @@ __global__ void k_viterbi(TreeNode* tree_out, const int32_t* tree_offsets, const
       const int c1 = c0 + (tn1 - tn0);
       const int c2 = c1 + (width == 2 ? tt1 - tt0 : 0);
       const int nnn = nn1 - nn0;
-      const int total = c2 + nnn * (width - 1);
       const float* nt_l = itop + (size_t)chart_index(begin, end - 1, n) * cell_size;
       const float* nt_r = ibot + (size_t)chart_index(end - 1, end, n) * cell_size;
       const float* tn_l = ibot + (size_t)chart_index(begin, begin + 1, n) * cell_size;
       const float* tn_r = itop + (size_t)chart_index(begin + 1, end, n) * cell_size;
       const float* tt_r = ibot + (size_t)chart_index(begin + 1, end, n) * cell_size;
+      // Candidates c: [0, c0) NT, [c0, c1) TN, [c1, c2) TT, then c2 + split * nnn + rule for NN.
       Best b{-300000.0f, -1};
-      for (int c = lane; c < total; c += 64) {
+      for (int c = tid; c < c2; c += VT) {
         float sc;
         if (c < c0) {
           const int r = nt0 + c;
@@ -1017,25 +1046,30 @@ __global__ void k_viterbi(TreeNode* tree_out, const int32_t* tree_offsets, const
         } else if (c < c1) {
           const int r = tn0 + (c - c0);
           sc = (scores[r] + tn_l[lefts[r]]) + tn_r[rights[r]];
-        } else if (c < c2) {
+        } else {
           const int r = tt0 + (c - c1);
           sc = (scores[r] + tn_l[lefts[r]]) + tt_r[rights[r]];
-        } else {
-          const int cc = c - c2;
-          const int si = cc / nnn;
-          const int r = nn0 + (cc - si * nnn);
-          const int split = begin + 1 + si;
-          const float* l = itop + (size_t)chart_index(begin, split, n) * cell_size;
-          const float* rr = itop + (size_t)chart_index(split, end, n) * cell_size;
-          sc = (scores[r] + l[lefts[r]]) + rr[rights[r]];
         }
         if (sc >= b.s) {
           b.s = sc;
           b.i = c;
         }
       }
-      b = wave_best(b);
-      if (lane == 0) {
+      for (int si = 0; si < width - 1; ++si) {
+        const int split = begin + 1 + si;
+        const float* l = itop + (size_t)chart_index(begin, split, n) * cell_size;
+        const float* rr = itop + (size_t)chart_index(split, end, n) * cell_size;
+        for (int j = tid; j < nnn; j += VT) {
+          const int r = nn0 + j;
+          const float sc = (scores[r] + l[lefts[r]]) + rr[rights[r]];
+          if (sc >= b.s) {
+            b.s = sc;
+            b.i = c2 + si * nnn + j;
+          }
+        }
+      }
+      b = block_best(b, s_best);
+      if (tid == 0) {
         int config, best_split, left = 10, right = 10;
         if (b.i < 0) {
           config = 0;
@@ -1084,7 +1118,7 @@ __global__ void k_viterbi(TreeNode* tree_out, const int32_t* tree_offsets, const
     __syncthreads();
   }
   TreeNode* out = tree_out + tree_offsets[sent];
-  for (int p = lane; p < nodes; p += 64) out[p] = TreeNode{s_top[p], s_bot[p], s_width[p], s_score[p]};
+  for (int p = tid; p < nodes; p += VT) out[p] = TreeNode{s_top[p], s_bot[p], s_width[p], s_score[p]};
 }
 
 __global__ void k_get_root(float* out, const float* inside, const int32_t* cell_offsets,
@@ -1964,7 +1998,9 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       pa.mask_words = g.mask_words;
       const dim3 pg((kc + 255) / 256, (unsigned)nfam);
       hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, stream_, pa);
-      hipLaunchKernelGGL(k_plan_sort, dim3(1, (unsigned)nfam), dim3(512), 0, stream_, pa);
+      bool any_key = false;
+      for (size_t fi = 0; fi < nfam; ++fi) any_key |= pa.fam[fi].key_blocks[0] >= 0;
+      if (any_key) hipLaunchKernelGGL(k_plan_sort, dim3(1, (unsigned)nfam), dim3(512), 0, stream_, pa);
       hipLaunchKernelGGL(k_plan_scan, dim3(1, (unsigned)nfam), dim3(1024), 0, stream_, pa);
       hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, stream_, pa);
       HIP_CHECK(hipGetLastError());
@@ -2302,7 +2338,7 @@ void Parser::launch_masks_and_scales(Batch& B, int32_t li, int32_t mask_words) {
 void Parser::launch_viterbi(Batch& B, int32_t li) {
   const Layer& L = *layers_[li];
   const Grammar& g = L.g;
-  hipLaunchKernelGGL(k_viterbi, dim3(B.n), dim3(64), 0, stream_, (TreeNode*)trees_.p,
+  hipLaunchKernelGGL(k_viterbi, dim3(B.n), dim3(kViterbiThreads), 0, stream_, (TreeNode*)trees_.p,
                      B.d_tree_offsets, (const float*)inside_.p, B.d_cell_offsets, B.d_lengths,
                      B.n, g.cell_size, g.root, g.num_nonterms(), L.d_dec_offsets,
                      L.d_dec_scores, L.d_dec_left, L.d_dec_right);
