@@ -1945,7 +1945,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     std::vector<int32_t> apps_slot(nfam, -1);
     int32_t* d_perm = nullptr;
     if (use_masks) {
-      if (kc >= (1 << 24)) throw std::runtime_error("mask plan slice exceeds 2^24 cells");
+      if (kc >= (1 << 23)) throw std::runtime_error("mask plan slice exceeds 2^23 cells");
       size_t total = 0;
       std::vector<size_t> oc(nfam), op(nfam), oi(nfam), oa(nfam), ok(nfam);
       for (size_t fi = 0; fi < nfam; ++fi) {
