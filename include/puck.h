@@ -27,7 +27,8 @@ extern "C" {
 typedef struct puck_grammar puck_grammar_t;
 typedef struct puck_parser puck_parser_t;
 
-enum puck_semiring { PUCK_VITERBI = 0, PUCK_REAL = 1 };
+/* RuleSemiring.scala:22-98: Viterbi (max, +), Real (+, x) with scaling, LogSum (log-add, +). */
+enum puck_semiring { PUCK_VITERBI = 0, PUCK_REAL = 1, PUCK_LOGSUM = 2 };
 
 /* Status codes. */
 enum puck_status {

@@ -18,7 +18,8 @@
  *   ref_viterbi    viterbi kernel (WG size 1)               CLViterbi.scala:364-605
  *   ref_mbr_cells  computeMBR kernel                        CLMBRKernels.scala:118-171
  *   ref_mbr_tree   extractMBRParses                         CLParser.scala:589-696
- * Semiring 0 = Viterbi (max, +), 1 = Real (+, *) (RuleSemiring.scala:22-98).
+ * Semiring 0 = Viterbi (max, +), 1 = Real (+, *), 2 = LogSum (log-add, +) (RuleSemiring.scala:22-98);
+ * LogSum's semiring_mad / semiring_add are restated from RuleSemiring.scala:58-73 (zero -1e6).
  * REAL is float by default; cky_ref64.c includes this file with REAL = double for the FP64 oracle.
  */
 #include <math.h>
@@ -31,6 +32,7 @@
 #define SUFFIX(x) x
 #define REF_FMA fmaf
 #define REF_EXP expf
+#define REF_LOG logf
 #endif
 
 typedef struct {
@@ -84,7 +86,15 @@ static int block_active(const RefGram* g, int which, int fam, const uint32_t* ma
   return 0;
 }
 
-static inline REAL zero_of(const RefGram* g) { return g->semiring == 0 ? (REAL)-INFINITY : (REAL)0; }
+static inline REAL zero_of(const RefGram* g) {
+  return g->semiring == 0 ? (REAL)-INFINITY : (g->semiring == 2 ? (REAL)-1000000.0 : (REAL)0);
+}
+
+/* LogSum semiring_add (RuleSemiring.scala:66-72): larger + log(1 + exp(smaller - larger)). */
+static inline REAL log_add(REAL x, REAL y) {
+  const REAL lo = x < y ? x : y, hi = x < y ? y : x;
+  return hi + (REAL)REF_LOG((REAL)1 + (REAL)REF_EXP(lo - hi));
+}
 
 /* Per-cell log scale; a missing scale array means every scale is 0 (PruningMask.scala:51-58). */
 #define SC(arr, i) ((arr) ? (REAL)(arr)[i] : (REAL)0)
@@ -97,7 +107,7 @@ void SUFFIX(ref_tags)(const RefGram* g, int n, const int32_t* words, const int32
   for (int pos = 0; pos < n; ++pos) {
     REAL* dst = chart + (size_t)bot_cell(pos, pos + 1, n) * g->cell_size;
     const int w = words[pos];
-    if (g->semiring == 0) {
+    if (g->semiring != 1) {
       for (int e = lex_off[w]; e < lex_off[w + 1]; ++e) dst[lex_tag[e]] = (REAL)(float)lex_logp[e];
     } else {
       const float my_scale = in_scale ? (float)exp(-(double)in_scale[bot_cell(pos, pos + 1, n)]) : 1.0f;
@@ -157,6 +167,8 @@ static void apply_binary(const RefGram* g, int which, int fam, int kind, REAL* t
       if (g->semiring == 0) {
         REAL v = (A[aa[k]] + B[bb[k]]) + q;
         if (v > tmp[t]) tmp[t] = v;
+      } else if (g->semiring == 2) {
+        tmp[t] = log_add(tmp[t], (A[aa[k]] + B[bb[k]]) + q);
       } else {
         tmp[t] = REF_FMA(A[aa[k]] * B[bb[k]], q, tmp[t]);
       }
@@ -164,6 +176,8 @@ static void apply_binary(const RefGram* g, int which, int fam, int kind, REAL* t
     if (g->semiring == 0) {
       for (int i = 0; i < cs; ++i)
         if (tmp[i] > tgt[i]) tgt[i] = tmp[i];
+    } else if (g->semiring == 2) {
+      for (int i = 0; i < cs; ++i) tgt[i] = log_add(tgt[i], tmp[i]);
     } else {
       const REAL sc = (REAL)REF_EXP(-SC(t_scale, tgt_cell) + SC(b_scale, bcell) + SC(a_scale, acell));
       for (int i = 0; i < cs; ++i) tgt[i] += tmp[i] * sc;
@@ -191,15 +205,19 @@ static void apply_unary(const RefGram* g, int which_rot, int fam, REAL* tgt_char
     if (g->semiring == 0) {
       REAL v = src[a] + q;
       if (v > tmp[t]) tmp[t] = v;
+    } else if (g->semiring == 2) {
+      tmp[t] = log_add(tmp[t], src[a] + q);
     } else {
       tmp[t] = REF_FMA(src[a], q, tmp[t]);
     }
   }
-  const REAL sc = g->semiring == 0 ? (REAL)1 : (REAL)REF_EXP(-SC(t_scale, tgt_cell) + SC(s_scale, src_cell));
+  const REAL sc = g->semiring != 1 ? (REAL)1 : (REAL)REF_EXP(-SC(t_scale, tgt_cell) + SC(s_scale, src_cell));
   for (int i = 0; i < cs; ++i) {
-    REAL v = g->semiring == 0 ? tmp[i] : tmp[i] * sc;
+    REAL v = g->semiring != 1 ? tmp[i] : tmp[i] * sc;
     if (overwrite) tgt[i] = v;
-    else tgt[i] = g->semiring == 0 ? (v > tgt[i] ? v : tgt[i]) : tgt[i] + v;
+    else if (g->semiring == 0) tgt[i] = v > tgt[i] ? v : tgt[i];
+    else if (g->semiring == 2) tgt[i] = log_add(tgt[i], v);
+    else tgt[i] = tgt[i] + v;
   }
   free(tmp);
 }
@@ -238,7 +256,7 @@ void SUFFIX(ref_outside)(const RefGram* g, int n, const REAL* in, REAL* out, con
   const int T = tri_(n);
   const int cs = g->cell_size;
   /* setRootScores */
-  out[(size_t)(T + cidx(0, n, n)) * cs + g->root] = g->semiring == 0 ? (REAL)0 : (REAL)1;
+  out[(size_t)(T + cidx(0, n, n)) * cs + g->root] = g->semiring == 1 ? (REAL)1 : (REAL)0;
   /* NU at the root span */
   apply_unary(g, 1, 0, out, cidx(0, n, n), out, T + cidx(0, n, n), out_scale, out_scale, masks, 0, n, n, 1);
   for (int w = n - 1; w >= 1; --w) {
@@ -410,7 +428,10 @@ void ref_mbr_cells(const RefGram* g, int n, const float* in, const float* out, i
     for (int c = 0; c < g->num_coarse; ++c) margs[c] = 0.0f;
     for (int sym = 0; sym < g->num_syms; ++sym) {
       const int c = sym < nproj ? proj[sym] : 0;
-      margs[c] += in[(size_t)cell * cs + sym] / root_score * out[(size_t)cell * cs + sym];
+      if (g->semiring == 2) /* CLMBRKernels.scala:146-150 */
+        margs[c] += expf(in[(size_t)cell * cs + sym] - root_score + out[(size_t)cell * cs + sym]);
+      else
+        margs[c] += in[(size_t)cell * cs + sym] / root_score * out[(size_t)cell * cs + sym];
     }
     float best = -INFINITY;
     int sym = -1;

@@ -7,5 +7,6 @@
 #define SUFFIX(x) x##_f64
 #define REF_FMA fma
 #define REF_EXP exp
+#define REF_LOG log
 #define REF_NO_DECODERS
 #include "cky_ref.c"

@@ -50,7 +50,8 @@ def _split(sym):
 
 
 class RefGrammar:
-    """Rule tables of one text grammar under one semiring ('viterbi' or 'real')."""
+    """Rule tables of one text grammar under one semiring ('viterbi', 'real' or 'logsum'; LogSum
+    keeps rule scores in log space like Viterbi, RuleSemiring.scala:45-63)."""
 
     def __init__(self, prefix, semiring="viterbi", threshold=THRESHOLD):
         self.prefix = prefix

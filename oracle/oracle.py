@@ -77,12 +77,12 @@ class Oracle:
         ref.load_lexicon()
         self.final_layer = final_layer
         self.cs = ref.cell_size
-        self.zero = -np.inf if ref.semiring == "viterbi" else 0.0
+        self.zero = {"viterbi": -np.inf, "real": 0.0, "logsum": -1e6}[ref.semiring]
         self._keep = []
         g = _Gram()
         g.nN, g.nT, g.cell_size = len(ref.nonterms), len(ref.terms), ref.cell_size
         g.num_syms, g.root, g.num_coarse = ref.num_syms, ref.root, len(ref.coarse)
-        g.mask_words, g.semiring = ref.mask_words, 0 if ref.semiring == "viterbi" else 1
+        g.mask_words, g.semiring = ref.mask_words, {"viterbi": 0, "real": 1, "logsum": 2}[ref.semiring]
         for f in range(4):
             b = ref.binaries[f]
             arrs = [np.ascontiguousarray(b[k]) for k in ("p", "l", "r", "q", "w")]
@@ -213,6 +213,12 @@ def pipeline(layers, sentences, f64_final=False):
             if L.ref.semiring == "viterbi":
                 rec["partition"] = np.float32(root)
                 rec["tree"] = L.viterbi(inside, n)
+            elif L.ref.semiring == "logsum":
+                # LogSumRuleSemiring.toLogSpace is the identity (RuleSemiring.scala:61); no scaling.
+                rec["partition"] = np.float32(root)
+                outside = L.outside(inside, n, masks)
+                rec["outside"] = outside
+                rec["mbr"], rec["mbr_cells"] = L.mbr(inside, outside, n, rec["in_scale"], rec["out_scale"], False)
             else:
                 top = np.float32(rec["in_scale"][tri(n) + cidx(0, n, n)])
                 rec["partition"] = np.float32(np.float32(np.log(np.float64(root))) + top) + np.float32(ps)
@@ -232,8 +238,9 @@ def brute_force_root(ref: RefGrammar, words, semiring="viterbi"):
     ref.load_lexicon()
     nN, nT = len(ref.nonterms), len(ref.terms)
     neg = -np.inf
-    add = (lambda a, b: np.maximum(a, b)) if semiring == "viterbi" else (lambda a, b: a + b)
-    zero = neg if semiring == "viterbi" else 0.0
+    logspace = semiring in ("viterbi", "logsum")
+    add = {"viterbi": np.maximum, "logsum": np.logaddexp, "real": lambda a, b: a + b}[semiring]
+    zero = neg if logspace else 0.0
     q = lambda arr: arr["q"].astype(np.float64)
 
     def tags(pos):
@@ -241,14 +248,14 @@ def brute_force_root(ref: RefGrammar, words, semiring="viterbi"):
         w = words[pos]
         for e in range(ref.lex_off[w], ref.lex_off[w + 1]):
             lp = float(np.float32(ref.lex_logp[e]))
-            v[ref.lex_tag[e]] = lp if semiring == "viterbi" else float(np.float32(np.exp(lp)))
+            v[ref.lex_tag[e]] = lp if logspace else float(np.float32(np.exp(lp)))
         return v
 
     def unary(vec, rules):
         out = np.full(nN, zero)
         for p, c, qq in zip(rules["p"], rules["c"], q(rules)):
-            val = vec[c] + qq if semiring == "viterbi" else vec[c] * qq
-            out[p] = max(out[p], val) if semiring == "viterbi" else out[p] + val
+            val = vec[c] + qq if logspace else vec[c] * qq
+            out[p] = add(out[p], val)
         return out
 
     def shapes(b, e):
@@ -278,8 +285,8 @@ def brute_force_root(ref: RefGrammar, words, semiring="viterbi"):
         for f, L, R in fams:
             rules = ref.binaries[f]
             for p, l, r, qq in zip(rules["p"], rules["l"], rules["r"], q(rules)):
-                val = (L[l] + R[r]) + qq if semiring == "viterbi" else L[l] * R[r] * qq
-                bot[p] = max(bot[p], val) if semiring == "viterbi" else bot[p] + val
+                val = (L[l] + R[r]) + qq if logspace else L[l] * R[r] * qq
+                bot[p] = add(bot[p], val)
         return bot, unary(bot, ref.unary_n), False
 
     best = zero

@@ -19,6 +19,7 @@ DATA_DIR = os.path.join(HERE, "data", "textGrammars")
 
 VITERBI = 0
 REAL = 1
+LOGSUM = 2
 
 _lib = None
 

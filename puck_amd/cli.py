@@ -2,8 +2,9 @@
 
   python -m puck_amd.cli compile --textGrammarPrefix a.gr:b.gr --grammar grammar.grz [--viterbi true]
       CompileGrammar.main (CompileGrammar.scala:19-74): every prefix is a layer, coarse first;
-      coarse layers use the Viterbi semiring, the last one Viterbi (--viterbi true) or Real
-      (--viterbi false, max-rule decoding). --logsum is not supported.
+      coarse layers use the Viterbi semiring, the last one Viterbi (--viterbi true), LogSum
+      (--viterbi false --logsum true) or Real (--viterbi false), the last two with max-rule
+      decoding.
   python -m puck_amd.cli parse --grammar grammar.grz [--maxLength 50] [--mem 16g] [files...]
       RunParser.main (RunParser.scala:31-138) with its "newline" sentence segmenter
       (NewLineSentenceSegmenter.scala:18-39) and "whitespace" tokenizer: one sentence per input
@@ -116,9 +117,13 @@ def parse_mem(s):
 def cmd_compile(args):
     import puck_amd as puck
     prefixes = args.textGrammarPrefix.split(":")
-    if args.logsum.lower() == "true":
-        raise SystemExit("the LogSum fine semiring is not supported; use --viterbi true or false")
-    fine = puck.VITERBI if args.viterbi.lower() == "true" else puck.REAL
+    # CompileGrammar.scala:46-52: Viterbi if --viterbi, else LogSum if --logsum, else Real.
+    if args.viterbi.lower() == "true":
+        fine = puck.VITERBI
+    elif args.logsum.lower() == "true":
+        fine = puck.LOGSUM
+    else:
+        fine = puck.REAL
     layers = [puck.Grammar.load_text(p, puck.VITERBI) for p in prefixes[:-1]]
     layers.append(puck.Grammar.load_text(prefixes[-1], fine))
     puck.save_grz(args.grammar, layers)

@@ -80,7 +80,8 @@ int32_t puck_abi_version(void) { return 1; }
 int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out) {
   return guarded([&]() -> int32_t {
     if (!prefix || !out) return set_error(PUCK_ERR_ARG, "null argument");
-    if (semiring != PUCK_VITERBI && semiring != PUCK_REAL) return set_error(PUCK_ERR_ARG, "unknown semiring");
+    if (semiring != PUCK_VITERBI && semiring != PUCK_REAL && semiring != PUCK_LOGSUM)
+      return set_error(PUCK_ERR_ARG, "unknown semiring");
     auto* h = new puck_grammar;
     try {
       h->g = std::make_shared<puck::Grammar>(puck::load_text_grammar(prefix, (puck::Semiring)semiring));
@@ -429,7 +430,7 @@ int32_t puck_server_create(const puck_grammar_t* const* layers, int32_t n_layers
       gs.push_back(layers[i]->g);
     }
     auto* h = new puck_server;
-    h->mbr = gs.back()->semiring == puck::kReal;
+    h->mbr = gs.back()->semiring != puck::kViterbi;
     try {
       h->s.reset(new puck::Server(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, max_inflight,
                                   max_batch > 0 ? max_batch : 20000, (flags & PUCK_FLAG_ONE_ITEM_PER_LANE) ? 1 : 2,

@@ -86,8 +86,27 @@ __device__ inline bool mask_any(const uint32_t* m, int words) {
 }
 
 template <int SEMI>
-__device__ inline float semi_zero() {
-  return SEMI == kViterbi ? -INFINITY : 0.0f;
+__host__ __device__ inline float semi_zero() {
+  return SEMI == kViterbi ? -INFINITY : (SEMI == kLogSum ? -1000000.0f : 0.0f);
+}
+
+inline float zero_of(int semiring) {
+  return semiring == kViterbi ? -INFINITY : (semiring == kLogSum ? -1000000.0f : 0.0f);
+}
+
+// LogSum semiring_add (RuleSemiring.scala:66-72): larger + log(1 + exp(smaller - larger)), with
+// the fast hardware exp/log as the reference's native_exp/native_log.
+__device__ inline float log_add(float x, float y) {
+  const float lo = fminf(x, y), hi = fmaxf(x, y);
+  return hi + __logf(1.0f + __expf(lo - hi));
+}
+
+// Semiring add of the reduce: max, +, or log-add.
+template <int SEMI>
+__device__ inline float semi_add(float acc, float v) {
+  if constexpr (SEMI == kViterbi) return fmaxf(acc, v);
+  else if constexpr (SEMI == kLogSum) return log_add(acc, v);
+  else return acc + v;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -376,7 +395,10 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       const float q = __uint_as_float(qs[u]);
 #pragma unroll
       for (int i = 0; i < IPL; ++i) {
-        if constexpr (UNARY) {
+        if constexpr (SEMI == kLogSum) {
+          // semiring_mad(x, y, z) = log_add(x, y + z) (RuleSemiring.scala:58-64)
+          acc[i] = UNARY ? log_add(acc[i], da[u][i] + q) : log_add(acc[i], (da[u][i] + db[u][i]) + q);
+        } else if constexpr (UNARY) {
           acc[i] = SEMI == kViterbi ? fmaxf(acc[i], da[u][i] + q) : fmaf(da[u][i], q, acc[i]);
         } else {
           acc[i] = SEMI == kViterbi ? fmaxf(acc[i], (da[u][i] + db[u][i]) + q)
@@ -691,18 +713,16 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
             if (SEMI == kViterbi) {
               acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
             } else {
-              acc += v0;
-              acc += v1;
-              acc += v2;
-              acc += v3;
+              acc = semi_add<SEMI>(acc, v0);
+              acc = semi_add<SEMI>(acc, v1);
+              acc = semi_add<SEMI>(acc, v2);
+              acc = semi_add<SEMI>(acc, v3);
             }
           }
         }
         for (; q < run; ++q) {
           const float* p0 = base + (size_t)a.slot[q] * pitch;
-          for (int j = lo; j < hi; ++j) {
-            if (SEMI == kViterbi) acc = fmaxf(acc, p0[j]); else acc += p0[j];
-          }
+          for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, p0[j]);
         }
       }
       e = run;
@@ -735,16 +755,13 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
             if (SEMI == kViterbi) {
               acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
             } else {
-              acc += v0;
-              acc += v1;
-              acc += v2;
-              acc += v3;
+              acc = semi_add<SEMI>(acc, v0);
+              acc = semi_add<SEMI>(acc, v1);
+              acc = semi_add<SEMI>(acc, v2);
+              acc = semi_add<SEMI>(acc, v3);
             }
           }
-          for (; j < uc; ++j) {
-            const float v = q[(size_t)j * a.num_cells];
-            if (SEMI == kViterbi) acc = fmaxf(acc, v); else acc += v;
-          }
+          for (; j < uc; ++j) acc = semi_add<SEMI>(acc, q[(size_t)j * a.num_cells]);
         }
         continue;
       }
@@ -771,8 +788,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int jz = use ? min(my_hi, pb + len) : 0;
         for (int j = max(my_lo, pb); j < jz; ++j) {
-          const float v = buf[j - pb];
-          if (SEMI == kViterbi) acc = fmaxf(acc, v); else acc += v;
+          acc = semi_add<SEMI>(acc, buf[j - pb]);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -824,7 +840,7 @@ __global__ void k_tags(float* inside, const int32_t* cell_offsets, const int32_t
   const int cell = cell_offsets[sent] + chart_index(pos, pos + 1, n);
   float* dst = inside + (size_t)cell * cell_size;
   const int w = words[p];
-  if (SEMI == kViterbi) {
+  if (SEMI != kReal) {
     for (int e = lex_off[w]; e < lex_off[w + 1]; ++e) dst[lex_tag[e]] = (float)lex_logp[e];
   } else {
     const float my_scale = (float)exp(-(double)in_scale[cell]);
@@ -894,7 +910,7 @@ __global__ void k_scaling(float* scale, const float* chart, int32_t num_syms, in
 __global__ void k_mbr(MbrCell* out, const float* inside, const float* outside,
                       const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
                       int32_t num_cells, const int32_t* proj_t, const int32_t* proj_n,
-                      int32_t num_syms, int32_t cell_size, int32_t root, int32_t num_coarse) {
+                      int32_t num_syms, int32_t cell_size, int32_t root, int32_t num_coarse, int32_t logsum) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= num_cells) return;
   int lo = 0, hi = num_sent;
@@ -911,7 +927,8 @@ __global__ void k_mbr(MbrCell* out, const float* inside, const float* outside,
   float margs[128];
   for (int c = 0; c < num_coarse; ++c) margs[c] = 0.0f;
   for (int sym = 0; sym < num_syms; ++sym) {
-    const float v = in[sym] / root_score * o[sym];
+    // CLMBRKernels.scala:145-150: exp(in - root + out) under LogSum, in / root * out otherwise.
+    const float v = logsum ? expf(in[sym] - root_score + o[sym]) : in[sym] / root_score * o[sym];
     margs[proj[sym]] += v;
   }
   MbrCell d{-INFINITY, -1};
@@ -1132,6 +1149,7 @@ template <class F>
 void launch_semi(int semi, F&& f) {
   if (semi == kViterbi) f(std::integral_constant<int, kViterbi>{});
   else if (semi == kReal) f(std::integral_constant<int, kReal>{});
+  else if (semi == kLogSum) f(std::integral_constant<int, kLogSum>{});
   else throw std::runtime_error("semiring not supported by the device engine");
 }
 
@@ -1314,7 +1332,7 @@ void Layer::build_family(int32_t id) {
   const int gw = block_major ? kSkipGroupWords : kGroupWords;
   std::vector<uint32_t> packed;  // kGroupWords per group
   std::vector<int32_t> wave_blocks;
-  const float null_q = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  const float null_q = zero_of(g.semiring);
   struct SRule { uint32_t x, y, q, tag; int32_t slot; };
   auto null_rule = [&]() {
     SRule u{0u, 0u, 0u, 0u, 0};
@@ -1554,6 +1572,13 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
                          (const void*)k_rules_q<1, kViterbi, false, false>, (const void*)k_rules_q<1, kViterbi, true, false>,
                          (const void*)k_rules_q<1, kViterbi, false, true>, (const void*)k_rules_q<1, kReal, false, false>,
                          (const void*)k_rules_q<1, kReal, true, false>, (const void*)k_rules_q<1, kReal, false, true>})
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  for (const void* fn : {(const void*)k_rules<2, kLogSum, false, false>, (const void*)k_rules<2, kLogSum, true, false>,
+                         (const void*)k_rules<2, kLogSum, false, true>, (const void*)k_rules<1, kLogSum, false, false>,
+                         (const void*)k_rules<1, kLogSum, true, false>, (const void*)k_rules<1, kLogSum, false, true>,
+                         (const void*)k_rules_q<2, kLogSum, false, false>, (const void*)k_rules_q<2, kLogSum, true, false>,
+                         (const void*)k_rules_q<2, kLogSum, false, true>, (const void*)k_rules_q<1, kLogSum, false, false>,
+                         (const void*)k_rules_q<1, kLogSum, true, false>, (const void*)k_rules_q<1, kLogSum, false, true>})
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   int cus = 0;
   HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
@@ -2077,7 +2102,7 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     ra.span = span;
     ra.root_init = (rid == kROutBin);
     ra.root = g.root;
-    ra.one = g.semiring == kViterbi ? 0.0f : 1.0f;
+    ra.one = g.semiring == kReal ? 1.0f : 0.0f;
     dim3 grid((kc + 63) / 64, (R.n_t + kReduceTargets - 1) / kReduceTargets);
     launch_semi(g.semiring, [&](auto semi) {
       constexpr int S = decltype(semi)::value;
@@ -2092,7 +2117,7 @@ void Parser::inside_pass(Batch& B, int32_t li, bool use_masks) {
   Layer& L = *layers_[li];
   const Grammar& g = L.g;
   float* inside = (float*)inside_.p;
-  const float zero = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  const float zero = zero_of(g.semiring);
   const size_t nfl = (size_t)B.num_cells * g.cell_size;
   hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, inside, nfl, zero);
   if (g.semiring == kReal && !B.has_scales) {
@@ -2117,7 +2142,7 @@ void Parser::inside_pass(Batch& B, int32_t li, bool use_masks) {
 void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
   Layer& L = *layers_[li];
   const Grammar& g = L.g;
-  const float zero = g.semiring == kViterbi ? -INFINITY : 0.0f;
+  const float zero = zero_of(g.semiring);
   const size_t nfl = (size_t)B.num_cells * g.cell_size;
   hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, (float*)outside_.p, nfl, zero);
   for (int w = B.max_len; w >= 1; --w) {
@@ -2351,7 +2376,7 @@ void Parser::launch_mbr(Batch& B, int32_t li) {
   hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
                      (MbrCell*)mbr_.p, (const float*)inside_.p, (const float*)outside_.p,
                      B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n,
-                     g.num_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size());
+                     g.num_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size(), g.semiring == kLogSum);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -2454,7 +2479,7 @@ void Parser::parse_batch(Batch& B, int32_t first, std::vector<std::vector<TreeNo
   }
   // Host state the device work leaves behind (also when it was replayed).
   B.has_masks = layers_.size() > 1;
-  B.has_scales = layers_.size() > 1 && !viterbi_final;
+  B.has_scales = layers_.size() > 1 && layers_.back()->g.semiring == kReal;
   finish_parse(B, first, viterbi, mbr);
   flush_timing();
 }
@@ -2596,7 +2621,7 @@ void Parser::parse(const int32_t* words, const int32_t* offsets, int32_t n,
   if (viterbi) viterbi->assign(n, {});
   if (mbr) mbr->assign(n, {});
   if (layers_.back()->g.semiring == kViterbi && !viterbi) throw std::runtime_error("viterbi output required");
-  if (layers_.back()->g.semiring == kReal && !mbr) throw std::runtime_error("mbr output required");
+  if (layers_.back()->g.semiring != kViterbi && !mbr) throw std::runtime_error("mbr output required");
   for (auto [f, c] : make_batches(offsets, n)) run_batch(words, offsets, f, c, 0, viterbi, mbr, nullptr, nullptr, 0, nullptr);
 }
 

@@ -319,11 +319,11 @@ void save_grz(const std::string& path, const std::vector<const Grammar*>& layers
   entries.push_back({"parserNames", names});
   for (size_t i = 0; i < layers.size(); ++i) {
     const Grammar& g = *layers[i];
-    if (g.semiring != kViterbi && g.semiring != kReal) throw std::runtime_error("grz stores Viterbi or Real layers");
+
     const std::string d = "parser-" + std::to_string(i) + "/";
     entries.push_back({d + "grammar", grammar_payload(g)});
     entries.push_back({d + "structure", structure_payload(g)});
-    entries.push_back({d + "semiring", g.semiring == kViterbi ? "Viterbi" : "Real"});
+    entries.push_back({d + "semiring", g.semiring == kViterbi ? "Viterbi" : (g.semiring == kReal ? "Real" : "LogSum")});
   }
   write_zip(path, entries);
 }
@@ -347,6 +347,7 @@ std::vector<std::shared_ptr<Grammar>> load_grz(const std::string& path) {
     const std::string& sr = get(d + "semiring");
     if (sr == "Viterbi") g->semiring = kViterbi;
     else if (sr == "Real") g->semiring = kReal;
+    else if (sr == "LogSum") g->semiring = kLogSum;
     else throw std::runtime_error("bad grz: semiring " + sr);
     out.push_back(g);
   }

@@ -10,7 +10,7 @@
 //   parser-i/grammar      symbols, coarse symbols, rule families with scores and kernel literals,
 //                         lexicon (see grz.cpp for the field order)
 //   parser-i/structure    CoarseParent block partitions and the Viterbi decoder tables
-//   parser-i/semiring     text: "Viterbi" or "Real"
+//   parser-i/semiring     text: "Viterbi", "Real" or "LogSum"
 // Device code is not stored: the rule streams are rebuilt from the tables when a parser is made.
 // Entries are stored uncompressed (zip method 0) with CRC-32 checks.
 #pragma once

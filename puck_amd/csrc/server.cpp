@@ -10,7 +10,7 @@ Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
                int32_t max_inflight, int32_t max_batch, int32_t items_per_lane, bool persistent)
     : max_batch_(std::max(1, max_batch)) {
   if (max_inflight < 1) throw std::runtime_error("max_inflight must be >= 1");
-  mbr_final_ = layers.back()->semiring == kReal;
+  mbr_final_ = layers.back()->semiring != kViterbi;
   const int64_t per_slot = mem_bytes / max_inflight;
   int32_t max_cell = 0;
   for (auto& g : layers) max_cell = std::max(max_cell, g->cell_size);

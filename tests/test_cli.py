@@ -52,3 +52,7 @@ def test_compile_writes_grz(tmp_path, puck):
     with zipfile.ZipFile(out) as z:
         assert z.read("parser-0/semiring") == b"Viterbi"
         assert z.read("parser-1/semiring") == b"Real"
+    cli.main(["compile", "--textGrammarPrefix", prefix, "--grammar", str(out), "--viterbi", "false",
+              "--logsum", "true"])
+    with zipfile.ZipFile(out) as z:
+        assert z.read("parser-1/semiring") == b"LogSum"

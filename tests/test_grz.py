@@ -62,6 +62,18 @@ def test_real_semiring_layer(tmp_path, puck, wsj0, wsj1_real):
     assert np.array_equal(qa, qb)
 
 
+def test_logsum_layer(tmp_path, puck, wsj0):
+    ls = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.LOGSUM)
+    path = str(tmp_path / "ls.grz")
+    puck.save_grz(path, [wsj0, ls])
+    with zipfile.ZipFile(path) as z:
+        assert z.read("parser-1/semiring") == b"LogSum"
+    back = puck.Grammar.load_grz(path)
+    assert back[1].info.semiring == puck.LOGSUM
+    for x, y in zip(ls.rules(0), back[1].rules(0)):
+        assert np.array_equal(x, y)
+
+
 def test_corrupted_and_truncated_files_are_rejected(tmp_path, grz, puck):
     data = bytearray(open(grz, "rb").read())
     bad = tmp_path / "bad.grz"

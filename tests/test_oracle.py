@@ -19,7 +19,7 @@ def test_chart_index_is_span_diagonal_major_bijection():
         assert cidx(3 % n, 0, n) == cidx(0, 3 % n, n) if n > 3 else True
 
 
-@pytest.mark.parametrize("semi", ["viterbi", "real"])
+@pytest.mark.parametrize("semi", ["viterbi", "real", "logsum"])
 @pytest.mark.parametrize("name", ["wsj_0", "wsj_1"])
 def test_inside_root_equals_brute_force(puck, semi, name):
     from oracle.grammar_ref import RefGrammar
@@ -34,6 +34,9 @@ def test_inside_root_equals_brute_force(puck, semi, name):
         bf = O.brute_force_root(ref, list(s), semi)
         if semi == "viterbi":
             assert math.isclose(root, bf, rel_tol=1e-6), (n, root, bf)
+        elif semi == "logsum":
+            # fp32 log-add in rule order vs exact logaddexp over whole trees
+            assert math.isclose(root, bf, rel_tol=1e-5), (n, root, bf)
         else:
             assert math.isclose(math.log(root) + ps, math.log(bf), rel_tol=1e-6), (n, root, bf)
 
@@ -51,6 +54,39 @@ def test_real_inside_outside_identity(puck, wsj1, ref_wsj1_real):
         for i in range(n):
             c = cidx(i, i + 1, n)
             assert math.isclose(float(np.dot(ins[c], out[c])), z, rel_tol=1e-9)
+
+
+def test_logsum_inside_outside_identity(puck, wsj1):
+    # In log space the same identity reads: logsumexp_t (in(i,t) + out(i,t)) = in(root) at every
+    # position i (FP64 oracle, log-add of RuleSemiring.scala:58-73).
+    from oracle.grammar_ref import RefGrammar
+    ref = RefGrammar(puck.grammar_path("wsj_1"), "logsum")
+    orc = O.Oracle(ref)
+    words, offsets = wsj1.sample(seed=6, n=4, mode=0, min_len=3, max_len=12)
+    for s in split_sentences(words, offsets):
+        n = len(s)
+        ins, _ = orc.inside(s, f64=True)
+        out = orc.outside(ins, n, f64=True)
+        z = ins[tri(n) + cidx(0, n, n), ref.root]
+        for i in range(n):
+            c = cidx(i, i + 1, n)
+            v = ins[c] + out[c]
+            assert math.isclose(float(np.logaddexp.reduce(v[v > -1e5])), z, rel_tol=1e-9)
+
+
+def test_logsum_matches_log_of_real_sum(puck, wsj1):
+    # LogSum's partition is log of the Real semiring's: the two differ only by the %f rounding
+    # of the rule literals (ln p vs p), far below this tolerance.
+    from oracle.grammar_ref import RefGrammar
+    ls = O.Oracle(RefGrammar(puck.grammar_path("wsj_1"), "logsum"))
+    re_ = O.Oracle(RefGrammar(puck.grammar_path("wsj_1"), "real"))
+    words, offsets = wsj1.sample(seed=8, n=4, mode=0, min_len=4, max_len=10)
+    for s in split_sentences(words, offsets):
+        n = len(s)
+        a, _ = ls.inside(s, f64=True)
+        b, ps = re_.inside(s, f64=True)
+        r = tri(n) + cidx(0, n, n)
+        assert abs(a[r, ls.ref.root] - (math.log(b[r, re_.ref.root]) + ps)) < 2e-3 * n
 
 
 def test_fp32_inside_close_to_fp64(wsj1, ref_wsj1_real):
