@@ -14,10 +14,9 @@
 #include <vector>
 
 #include "grammar.h"
+#include "kernels.h"
 
 namespace puck {
-
-struct FamArgs;
 
 // One node of a Viterbi tree, pre-order, 2n-1 nodes per sentence (CLViterbi.scala:378).
 struct TreeNode {
@@ -223,6 +222,7 @@ class Parser {
   int32_t device_;
   int64_t mem_bytes_;
   hipStream_t stream_ = nullptr;
+  hipStream_t plan_stream_ = nullptr;  // mask plans of a masked pass, ahead of its rule launches
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // apps_slot >= 0: the launch's rule applications add the plan counts in
   // apps_[apps_slot, apps_slot + apps_count).
@@ -278,7 +278,24 @@ class Parser {
                     const std::vector<const int32_t*>& plan_counts,
                     const std::vector<const int32_t*>& plan_items,
                     const std::vector<const uint32_t*>& plan_acts, const std::vector<int32_t>& apps_slots);
-  void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks);
+  // Mask plan of one slice of a group's cells at one span.
+  struct PlannedSlice {
+    int32_t k0 = 0, kc = 0;
+    std::vector<PlanFam> pfs;
+    std::vector<int32_t> apps_slot;
+    int32_t* d_perm = nullptr;
+    hipEvent_t ready = nullptr;  // recorded on plan_stream_ after the slice's plan kernels
+  };
+  std::vector<std::pair<int32_t, int32_t>> group_slices(Batch& b, int32_t li, int32_t rid, int32_t span);
+  int64_t family_items(Batch& b, int32_t li, int32_t fid, int32_t span, int32_t k0, int32_t kc);
+  size_t plan_ints(Batch& b, int32_t li, int32_t rid, int32_t span, int32_t k0, int32_t kc);
+  PlannedSlice plan_slice(Batch& b, int32_t li, int32_t rid, int32_t span, int32_t k0, int32_t kc,
+                          int32_t* mp, hipStream_t st);
+  void plan_pass(Batch& b, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
+                 std::vector<std::vector<PlannedSlice>>* out);
+  void run_groups(Batch& b, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups, bool use_masks);
+  void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks,
+                 const std::vector<PlannedSlice>* planned);
   void timed_begin();
   void timed_end(double* acc);
 };

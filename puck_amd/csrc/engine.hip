@@ -1555,6 +1555,7 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   // Non-blocking: work of other parsers' streams (and of the null stream) never orders against
   // this one, so parsers on one device run concurrently.
   HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&plan_stream_, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreate(&ev0_));
   HIP_CHECK(hipEventCreate(&ev1_));
   // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
@@ -1593,6 +1594,7 @@ Parser::~Parser() {
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  if (plan_stream_) (void)hipStreamDestroy(plan_stream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -1916,36 +1918,13 @@ void Parser::flush_timing() {
   ev_next_ = 0;
 }
 
-void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use_masks) {
+// Slices of a group's cells at one span whose partials fit the partial buffer (num_slots x pitch
+// floats per family, pitch = items rounded up to 256).
+std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t li, int32_t rid, int32_t span) {
   Layer& L = *layers_[li];
   const DevReduce& R = L.red[rid];
-  const Grammar& g = L.g;
   const int32_t ncells = B.span_count[span];
-  if (ncells == 0) return;
-  const size_t nfam = R.families.size();
-  // Items per cell for each family of the group, unpruned (an upper bound under masks).
-  auto cell_items = [&](int fid, int k) -> int64_t {
-    const DevFamily& F = L.fam[fid];
-    if (uniform_kind(F.kind)) return F.kind == 0 ? span - 1 : 1;
-    const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
-    return B.h_prefix[po + k + 1] - B.h_prefix[po + k];
-  };
-  auto slice_items = [&](int fid, int k0, int kc) -> int64_t {
-    const DevFamily& F = L.fam[fid];
-    if (uniform_kind(F.kind)) return (int64_t)kc * (F.kind == 0 ? span - 1 : 1);
-    const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
-    return B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
-  };
-  const uint32_t* masks = use_masks ? (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words
-                                    : nullptr;
-  // Under masks every family orders the cells by its own key; the reduce visits them in the order
-  // of the family with most rules.
-  int key_fam = 0;
-  for (size_t fi = 1; fi < nfam; ++fi)
-    if (L.fam[R.families[fi]].num_rules > L.fam[R.families[key_fam]].num_rules) key_fam = (int)fi;
-
-  // Slices of cells whose partials fit the partial buffer (num_slots x pitch floats per family,
-  // pitch = items rounded up to 256).
+  std::vector<std::pair<int32_t, int32_t>> out;
   const int64_t budget_floats = (int64_t)(partial_.bytes / 4);
   int64_t fixed = 0;
   for (int fid : R.families) fixed += 256 * (int64_t)L.fam[fid].num_slots;
@@ -1955,81 +1934,165 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
     int64_t used = 0;
     while (k1 < ncells) {
       int64_t add = 0;
-      for (int fid : R.families) add += cell_items(fid, k1) * L.fam[fid].num_slots;
+      for (int fid : R.families) add += family_items(B, li, fid, span, k1, 1) * L.fam[fid].num_slots;
       if (k1 > k0 && used + add + fixed > budget_floats) break;
       used += add;
       ++k1;
     }
     if (used + fixed > budget_floats) throw std::runtime_error("partial buffer too small for one cell");
-    const int32_t kc = k1 - k0;
-    const int32_t* cells = B.d_cells + B.span_start[span] + k0;
+    out.push_back({k0, k1 - k0});
+    k0 = k1;
+  }
+  return out;
+}
 
-    // Mask plan of the slice (device): per family counts, active-block bits and the rule
-    // applications, the family's cell order, and the prefix and item list in it.
-    std::vector<PlanFam> pfs(nfam);
-    std::vector<int32_t> apps_slot(nfam, -1);
-    int32_t* d_perm = nullptr;
-    if (use_masks) {
-      if (kc >= (1 << 23)) throw std::runtime_error("mask plan slice exceeds 2^23 cells");
-      size_t total = 0;
-      std::vector<size_t> oc(nfam), op(nfam), oi(nfam), oa(nfam), ok(nfam);
-      for (size_t fi = 0; fi < nfam; ++fi) {
-        ok[fi] = total;  // keys, perm, posof
-        total += 3 * (size_t)kc;
-        oc[fi] = total;
-        total += (size_t)kc;
-        op[fi] = total;
-        total += (size_t)kc + 1;
-        oa[fi] = total;
-        total += (size_t)4 * kc;
-        oi[fi] = total;
-        total += (size_t)std::max<int64_t>(slice_items(R.families[fi], k0, kc), 1);
-      }
-      ensure(mplan_, total * 4);
-      int32_t* mp = (int32_t*)mplan_.p;
-      PlanArgs pa{};
-      for (size_t fi = 0; fi < nfam; ++fi) {
-        const DevFamily& F = L.fam[R.families[fi]];
-        PlanFam& pf = pa.fam[fi];
-        pf.kind = F.kind;
-        pf.unary = F.unary;
-        pf.a_half = F.a_half;
-        pf.b_half = F.b_half;
-        pf.t_half = F.t_half;
-        pf.num_blocks = F.unary ? 0 : F.num_blocks;
-        pf.block_masks = F.unary ? nullptr : F.d_block_masks;
-        pf.counts = mp + oc[fi];
-        pf.pre = mp + op[fi];
-        pf.items = mp + oi[fi];
-        pf.act = (uint32_t*)(mp + oa[fi]);
-        pf.keys = mp + ok[fi];
-        pf.perm = mp + ok[fi] + kc;
-        pf.posof = mp + ok[fi] + 2 * (size_t)kc;
-        for (int i = 0; i < kPlanKeyBlocks; ++i)
-          pf.key_blocks[i] = i < (int)F.h_key_blocks.size() ? F.h_key_blocks[i] : -1;
-        pf.block_rules = F.d_block_rules;
-        if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
-        apps_slot[fi] = apps_next_;
-        pf.apps = (double*)apps_.p + apps_next_++;
-        pfs[fi] = pf;
-      }
-      d_perm = pa.fam[key_fam].perm;
-      pa.masks = masks;
-      pa.cell_offsets = B.d_cell_offsets;
-      pa.lengths = B.d_lengths;
-      pa.cell_list = cells;
-      pa.num_cells = kc;
-      pa.span = span;
-      pa.mask_words = g.mask_words;
-      const dim3 pg((kc + 255) / 256, (unsigned)nfam);
-      hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, stream_, pa);
-      bool any_key = false;
-      for (size_t fi = 0; fi < nfam; ++fi) any_key |= pa.fam[fi].key_blocks[0] >= 0;
-      if (any_key) hipLaunchKernelGGL(k_plan_sort, dim3(1, (unsigned)nfam), dim3(512), 0, stream_, pa);
-      hipLaunchKernelGGL(k_plan_scan, dim3(1, (unsigned)nfam), dim3(1024), 0, stream_, pa);
-      hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, stream_, pa);
-      HIP_CHECK(hipGetLastError());
+// Items of cells [k0, k0 + kc) of a span for one family, unpruned (an upper bound under masks).
+int64_t Parser::family_items(Batch& B, int32_t li, int32_t fid, int32_t span, int32_t k0, int32_t kc) {
+  const DevFamily& F = layers_[li]->fam[fid];
+  if (uniform_kind(F.kind)) return (int64_t)kc * (F.kind == 0 ? span - 1 : 1);
+  const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+  return B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
+}
+
+// Ints of mask-plan memory one slice needs: per family keys, perm, posof, counts, prefix,
+// active-block bits and the item list.
+size_t Parser::plan_ints(Batch& B, int32_t li, int32_t rid, int32_t span, int32_t k0, int32_t kc) {
+  size_t total = 0;
+  for (int fid : layers_[li]->red[rid].families)
+    total += 9 * (size_t)kc + 1 + (size_t)std::max<int64_t>(family_items(B, li, fid, span, k0, kc), 1);
+  return total;
+}
+
+// Mask plan of one slice (device): per family counts, active-block bits and the rule applications,
+// the family's cell order, and the prefix and item list in it. Launches on stream `st`, in `mp`.
+Parser::PlannedSlice Parser::plan_slice(Batch& B, int32_t li, int32_t rid, int32_t span, int32_t k0,
+                                        int32_t kc, int32_t* mp, hipStream_t st) {
+  Layer& L = *layers_[li];
+  const DevReduce& R = L.red[rid];
+  const Grammar& g = L.g;
+  const size_t nfam = R.families.size();
+  if (kc >= (1 << 23)) throw std::runtime_error("mask plan slice exceeds 2^23 cells");
+  PlannedSlice ps;
+  ps.k0 = k0;
+  ps.kc = kc;
+  ps.pfs.resize(nfam);
+  ps.apps_slot.assign(nfam, -1);
+  int key_fam = 0;
+  for (size_t fi = 1; fi < nfam; ++fi)
+    if (L.fam[R.families[fi]].num_rules > L.fam[R.families[key_fam]].num_rules) key_fam = (int)fi;
+  size_t o = 0;
+  PlanArgs pa{};
+  for (size_t fi = 0; fi < nfam; ++fi) {
+    const DevFamily& F = L.fam[R.families[fi]];
+    PlanFam& pf = pa.fam[fi];
+    pf.kind = F.kind;
+    pf.unary = F.unary;
+    pf.a_half = F.a_half;
+    pf.b_half = F.b_half;
+    pf.t_half = F.t_half;
+    pf.num_blocks = F.unary ? 0 : F.num_blocks;
+    pf.block_masks = F.unary ? nullptr : F.d_block_masks;
+    pf.keys = mp + o;
+    pf.perm = mp + o + kc;
+    pf.posof = mp + o + 2 * (size_t)kc;
+    o += 3 * (size_t)kc;
+    pf.counts = mp + o;
+    o += (size_t)kc;
+    pf.pre = mp + o;
+    o += (size_t)kc + 1;
+    pf.act = (uint32_t*)(mp + o);
+    o += (size_t)4 * kc;
+    pf.items = mp + o;
+    o += (size_t)std::max<int64_t>(family_items(B, li, R.families[fi], span, k0, kc), 1);
+    for (int i = 0; i < kPlanKeyBlocks; ++i)
+      pf.key_blocks[i] = i < (int)F.h_key_blocks.size() ? F.h_key_blocks[i] : -1;
+    pf.block_rules = F.d_block_rules;
+    if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
+    ps.apps_slot[fi] = apps_next_;
+    pf.apps = (double*)apps_.p + apps_next_++;
+    ps.pfs[fi] = pf;
+  }
+  ps.d_perm = pa.fam[key_fam].perm;
+  pa.masks = (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words;
+  pa.cell_offsets = B.d_cell_offsets;
+  pa.lengths = B.d_lengths;
+  pa.cell_list = B.d_cells + B.span_start[span] + k0;
+  pa.num_cells = kc;
+  pa.span = span;
+  pa.mask_words = g.mask_words;
+  const dim3 pg((kc + 255) / 256, (unsigned)nfam);
+  hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, st, pa);
+  bool any_key = false;
+  for (size_t fi = 0; fi < nfam; ++fi) any_key |= pa.fam[fi].key_blocks[0] >= 0;
+  if (any_key) hipLaunchKernelGGL(k_plan_sort, dim3(1, (unsigned)nfam), dim3(512), 0, st, pa);
+  hipLaunchKernelGGL(k_plan_scan, dim3(1, (unsigned)nfam), dim3(1024), 0, st, pa);
+  hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, st, pa);
+  HIP_CHECK(hipGetLastError());
+  return ps;
+}
+
+// Plans every (span, group) of a masked pass up front on the plan stream, so that the small plan
+// kernels of later spans run beside the rule and reduce launches of earlier ones; each group's
+// rule launch waits for its own plan only.
+void Parser::plan_pass(Batch& B, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
+                       std::vector<std::vector<PlannedSlice>>* out) {
+  size_t total = 0;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> slices(groups.size());
+  for (size_t k = 0; k < groups.size(); ++k) {
+    const auto [rid, span] = groups[k];
+    if (B.span_count[span] == 0) continue;
+    slices[k] = group_slices(B, li, rid, span);
+    for (auto [k0, kc] : slices[k]) total += plan_ints(B, li, rid, span, k0, kc);
+  }
+  ensure(mplan_, std::max<size_t>(total, 1) * 4);
+  hipEvent_t fork = next_event();
+  HIP_CHECK(hipEventRecord(fork, stream_));
+  HIP_CHECK(hipStreamWaitEvent(plan_stream_, fork, 0));
+  int32_t* mp = (int32_t*)mplan_.p;
+  out->assign(groups.size(), {});
+  for (size_t k = 0; k < groups.size(); ++k) {
+    const auto [rid, span] = groups[k];
+    for (auto [k0, kc] : slices[k]) {
+      PlannedSlice ps = plan_slice(B, li, rid, span, k0, kc, mp, plan_stream_);
+      mp += plan_ints(B, li, rid, span, k0, kc);
+      ps.ready = next_event();
+      HIP_CHECK(hipEventRecord(ps.ready, plan_stream_));
+      (*out)[k].push_back(std::move(ps));
     }
+  }
+}
+
+void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use_masks,
+                       const std::vector<PlannedSlice>* planned) {
+  Layer& L = *layers_[li];
+  const DevReduce& R = L.red[rid];
+  const Grammar& g = L.g;
+  const int32_t ncells = B.span_count[span];
+  if (ncells == 0) return;
+  const size_t nfam = R.families.size();
+  auto slice_items = [&](int fid, int k0, int kc) -> int64_t { return family_items(B, li, fid, span, k0, kc); };
+  const auto slices = group_slices(B, li, rid, span);
+  if (planned != nullptr && planned->size() != slices.size()) throw std::logic_error("plan slices differ");
+  for (size_t si = 0; si < slices.size(); ++si) {
+    const int32_t k0 = slices[si].first, kc = slices[si].second;
+    const int32_t* cells = B.d_cells + B.span_start[span] + k0;
+    PlannedSlice own;
+    const PlannedSlice* ps = nullptr;
+    if (use_masks) {
+      if (planned != nullptr) {
+        ps = &(*planned)[si];
+        HIP_CHECK(hipStreamWaitEvent(stream_, ps->ready, 0));
+      } else {
+        ensure(mplan_, plan_ints(B, li, rid, span, k0, kc) * 4);
+        own = plan_slice(B, li, rid, span, k0, kc, (int32_t*)mplan_.p, stream_);
+        ps = &own;
+      }
+    }
+    const std::vector<PlanFam> empty;
+    const std::vector<PlanFam>& pfs = ps ? ps->pfs : empty;
+    const std::vector<int32_t> no_slots(nfam, -1);
+    const std::vector<int32_t>& apps_slot = ps ? ps->apps_slot : no_slots;
+    int32_t* d_perm = ps ? ps->d_perm : nullptr;
 
     RedArgs ra{};
     float* base = (float*)partial_.p;
@@ -2109,7 +2172,6 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       hipLaunchKernelGGL((k_reduce<S>), grid, dim3(64 * kReduceTargets), 0, stream_, ra);
     });
     HIP_CHECK(hipGetLastError());
-    k0 = k1;
   }
 }
 
@@ -2132,11 +2194,14 @@ void Parser::inside_pass(Batch& B, int32_t li, bool use_masks) {
                        (double*)pos_scale_.p, g.cell_size);
   });
   HIP_CHECK(hipGetLastError());
-  run_group(B, li, kRInUT, 1, use_masks);
+  // The pass's groups in schedule order (CLParser.scala:256-314): TU at span 1, then per span the
+  // binary families and NU.
+  std::vector<std::pair<int32_t, int32_t>> groups{{kRInUT, 1}};
   for (int w = 2; w <= B.max_len; ++w) {
-    run_group(B, li, w == 2 ? kRInBin2 : kRInBin, w, use_masks);
-    run_group(B, li, kRInUN, w, use_masks);
+    groups.push_back({w == 2 ? kRInBin2 : kRInBin, w});
+    groups.push_back({kRInUN, w});
   }
+  run_groups(B, li, groups, use_masks);
 }
 
 void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
@@ -2145,10 +2210,29 @@ void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
   const float zero = zero_of(g.semiring);
   const size_t nfl = (size_t)B.num_cells * g.cell_size;
   hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, (float*)outside_.p, nfl, zero);
+  // CLParser.scala:316-369: per span from the root down, the binary families then NU (TU at 1).
+  std::vector<std::pair<int32_t, int32_t>> groups;
   for (int w = B.max_len; w >= 1; --w) {
-    run_group(B, li, kROutBin, w, use_masks);
-    run_group(B, li, w >= 2 ? kROutUN : kROutW1, w, use_masks);
+    groups.push_back({kROutBin, w});
+    groups.push_back({w >= 2 ? kROutUN : kROutW1, w});
   }
+  run_groups(B, li, groups, use_masks);
+}
+
+// Runs a pass's (group, span) list in order. Under masks the plans of every group are made first
+// on the plan stream (plan_pass), and the plan stream is joined back at the end.
+void Parser::run_groups(Batch& B, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
+                        bool use_masks) {
+  if (!use_masks) {
+    for (auto [rid, w] : groups) run_group(B, li, rid, w, false, nullptr);
+    return;
+  }
+  std::vector<std::vector<PlannedSlice>> plans;
+  plan_pass(B, li, groups, &plans);
+  for (size_t k = 0; k < groups.size(); ++k) run_group(B, li, groups[k].first, groups[k].second, true, &plans[k]);
+  hipEvent_t join = next_event();
+  HIP_CHECK(hipEventRecord(join, plan_stream_));
+  HIP_CHECK(hipStreamWaitEvent(stream_, join, 0));
 }
 
 // what: 0 = parse, 1 = partitions, 2 = insideOutside, 3 = masks of the last coarse layer,
