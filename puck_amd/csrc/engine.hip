@@ -101,6 +101,14 @@ __device__ inline float log_add(float x, float y) {
   return hi + __logf(1.0f + __expf(lo - hi));
 }
 
+// Semiring add of a lane's two paired splits in the rule kernel's flush.
+template <int SEMI>
+__device__ inline float semi_pair(float a, float b) {
+  if constexpr (SEMI == kViterbi) return fmaxf(a, b);
+  else if constexpr (SEMI == kLogSum) return log_add(a, b);
+  else return a + b;
+}
+
 // Semiring add of the reduce: max, +, or log-add.
 template <int SEMI>
 __device__ inline float semi_add(float acc, float v) {
@@ -165,12 +173,21 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     int ca = 0, cb = 0;
     uint32_t gate[4] = {0u, 0u, 0u, 0u};
     float scale = 1.0f;
-    if (item < num_items) {
-      int k, j;
+    int k = 0, j = 0;
+    bool valid = item < num_items;
+    if (valid) {
       if (a.plan_items != nullptr) {
         const int code = a.plan_items[item_begin + item];
         k = (code >> 8) - a.slice_cell0;
         j = code & 255;
+      } else if (a.prefix == nullptr && a.pair_splits) {
+        // Paired split-major items: items 2m and 2m+1 (one lane) are splits 2p and 2p+1 of one
+        // cell, pair m = p * num_cells + cell; a lane folds its pair before writing the partial.
+        const int m = item >> 1;
+        const int pr = m / a.num_cells;
+        k = m - pr * a.num_cells;
+        j = 2 * pr + (item & 1);
+        valid = j < a.span - 1;
       } else if (a.prefix == nullptr) {
         // Uniform families are split-major: item = split * num_cells + cell, so that the reduce
         // reads each split's partials of consecutive cells as one coalesced row.
@@ -186,6 +203,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
         k = lo;
         j = abs_item - a.prefix[k];
       }
+    }
+    if (valid) {
       const int packed = a.cell_list[k];
       const int sent = packed >> 8, b = packed & 255, e = b + a.span;
       const int n = a.lengths[sent];
@@ -379,6 +398,21 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
     const int bit = (int)((tag >> 24) & 127u);
     float* out = a.partial + (size_t)slot * a.pitch;
+    if constexpr (IPL == 2) {
+      if (a.pair_splits) {
+        float v[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
+          v[i] = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
+          if (SEMI == kReal) v[i] *= scl[i];
+          acc[i] = zero;
+        }
+        const int m = (tile0 >> 1) + lane;
+        if (2 * m < num_items) out[m] = semi_pair<SEMI>(v[0], v[1]);
+        return;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < IPL; ++i) {
       const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
@@ -1670,6 +1704,12 @@ void Parser::timed_end(double* acc) {
 
 namespace {
 bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
+// Unplanned inside NN launches with two items per lane fold split pairs in the rule kernel when
+// the splits pair up without a lone last split or that lone split is a small share of the work.
+bool pairs_splits(const Layer& L, const DevFamily& F, int span) {
+  const int splits = span - 1;
+  return L.ipl == 2 && F.kind == 0 && splits >= 2 && (splits % 2 == 0 || splits >= 15);
+}
 }  // namespace
 
 bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool use_masks, int32_t k0,
@@ -1711,6 +1751,12 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
     a.prefix = nullptr;
     a.uniform_count = F.kind == 0 ? span - 1 : 1;
     nitems = (int64_t)kc * a.uniform_count;
+    if (pairs_splits(L, F, span)) {
+      // Items 2m, 2m+1 are splits 2p, 2p+1 of one cell; a partial row holds one value per pair.
+      a.pair_splits = 1;
+      a.uniform_count = span / 2;  // pairs: ceil((span - 1) / 2)
+      nitems = 2 * (int64_t)kc * a.uniform_count;
+    }
   } else {
     const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
     a.prefix = B.d_prefix + po + k0;
@@ -1729,7 +1775,7 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.num_blocks = F.num_blocks;
   a.partial = partial;
   const int items_tile = 64 * L.ipl;
-  a.pitch = (int32_t)((nitems + 255) / 256 * 256);
+  a.pitch = (int32_t)(((a.pair_splits ? nitems / 2 : nitems) + 255) / 256 * 256);
   a.span = span;
   a.kind = F.kind;
   a.range_split = F.range_split;
@@ -1783,7 +1829,8 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   HIP_CHECK(hipGetLastError());
   // Planned launches take their count from the plan (the first slice of a span carries the whole
   // span's count); others count every rule for every item.
-  const double apps = plan_items != nullptr ? 0.0 : (double)a.num_items * F.rule_apps_per_item;
+  const double apps =
+      plan_items != nullptr ? 0.0 : (double)family_items(B, li, fid, span, k0, kc) * F.rule_apps_per_item;
   stats.rule_apps += apps;
   if (!timing) return;
   HIP_CHECK(hipEventRecord(t1, stream_));
@@ -1821,7 +1868,8 @@ void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids
     q.fam[q.nfam++] = a;
     lds = std::max(lds, l);
     units += (int64_t)grid.x * grid.y * grid.z;
-    if (plan_items[i] == nullptr) apps += (double)a.num_items * L.fam[fids[i]].rule_apps_per_item;
+    if (plan_items[i] == nullptr)
+      apps += (double)family_items(B, li, fids[i], span, k0, kc) * L.fam[fids[i]].rule_apps_per_item;
   }
   if (q.nfam == 0) return;
   if (units >= (int64_t)1 << 31) throw std::runtime_error("rule work queue exceeds 2^31 units");
@@ -2117,7 +2165,12 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
         ra.prefix[fi] = B.d_prefix + po + k0;
         ra.uniform_count[fi] = 1;
       }
-      const int32_t pitch = (int32_t)((nitems + 255) / 256 * 256);
+      int64_t row_items = nitems;  // partial values per slot row
+      if (!use_masks && pairs_splits(L, F, span)) {
+        ra.uniform_count[fi] = span / 2;
+        row_items = (int64_t)kc * (span / 2);
+      }
+      const int32_t pitch = (int32_t)((row_items + 255) / 256 * 256);
       ra.partial[fi] = base + off;
       fam_partial[fi] = base + off;
       ra.pitch[fi] = pitch;

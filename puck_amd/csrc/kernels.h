@@ -62,6 +62,7 @@ struct FamArgs {
   int32_t pitch;
   int32_t row_stride;  // floats per LDS row
   int32_t num_chunks;  // operand chunks of the family (grid y of the plain launch)
+  int32_t pair_splits; // uniform NN: items 2m, 2m+1 are splits 2p, 2p+1 of one cell (pair m)
 };
 
 // Arguments of the persistent rule kernel: the families of one span that share a layout.
