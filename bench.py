@@ -170,30 +170,37 @@ def serve(puck, layers, pool, rate, seconds, seed):
             "mean_batch": round(st["mean_batch"], 1)}
 
 
-def side_workloads(puck, g6v, prefix, args, cfg):
+SIDE = ("c2f_viterbi", "c2f_mbr", "mixed_lengths", "serving")
+
+
+def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
     """Coarse-to-fine, mixed-length and serving workloads, measured beside the headline."""
     out = {}
     g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
     words, offsets = g6v.sample(seed=3, n=args.c2f_sentences, mode=0, min_len=1, max_len=40)
-    out["c2f_viterbi"] = plain_vs_batched(
-        puck, [g1, g6v], words, offsets, 2, cfg,
-        f"wsj_1 -> synthetic wsj_6-shape Viterbi, masks at -9, {args.c2f_sentences} PCFG sentences 1-40")
-    g6r = puck.Grammar.load_text(prefix, puck.REAL)
-    out["c2f_mbr"] = plain_vs_batched(
-        puck, [g1, g6r], words, offsets, 2, cfg,
-        f"wsj_1 -> synthetic wsj_6-shape Real inside/outside + MBR, {args.c2f_sentences} PCFG sentences 1-40")
-    del g6r
-    mw, mo = g6v.sample(seed=4, n=args.mixed_sentences, mode=2, min_len=1, max_len=80)
-    out["mixed_lengths"] = plain_vs_batched(
-        puck, [g1, g6v], mw, mo, 1, cfg,
-        f"c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
-    pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
-    pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
-    out["serving"] = []
-    for rate in (200, 1000, 5000):
-        r = serve(puck, [g1, g6v], pool, rate, args.serve_seconds, seed=rate)
-        out["serving"].append(r)
-        print(f"[bench] serving {r}", file=sys.stderr, flush=True)
+    if "c2f_viterbi" in which:
+        out["c2f_viterbi"] = plain_vs_batched(
+            puck, [g1, g6v], words, offsets, 2, cfg,
+            f"wsj_1 -> synthetic wsj_6-shape Viterbi, masks at -9, {args.c2f_sentences} PCFG sentences 1-40")
+    if "c2f_mbr" in which:
+        g6r = puck.Grammar.load_text(prefix, puck.REAL)
+        out["c2f_mbr"] = plain_vs_batched(
+            puck, [g1, g6r], words, offsets, 2, cfg,
+            f"wsj_1 -> synthetic wsj_6-shape Real inside/outside + MBR, {args.c2f_sentences} PCFG sentences 1-40")
+        del g6r
+    if "mixed_lengths" in which:
+        mw, mo = g6v.sample(seed=4, n=args.mixed_sentences, mode=2, min_len=1, max_len=80)
+        out["mixed_lengths"] = plain_vs_batched(
+            puck, [g1, g6v], mw, mo, 1, cfg,
+            f"c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
+    if "serving" in which:
+        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
+        pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
+        out["serving"] = []
+        for rate in (200, 1000, 5000):
+            r = serve(puck, [g1, g6v], pool, rate, args.serve_seconds, seed=rate)
+            out["serving"].append(r)
+            print(f"[bench] serving {r}", file=sys.stderr, flush=True)
     return out
 
 
@@ -221,6 +228,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--check", type=int, default=8, help="sentences whose trees are checked against the oracle")
     ap.add_argument("--side", type=int, default=1, help="also run the c2f, mixed-length and serving workloads")
+    ap.add_argument("--only", default="", help="comma list of side workloads to run alone, without the headline "
+                                               "(" + ",".join(SIDE) + "); for profiling")
     ap.add_argument("--c2f-sentences", type=int, default=5000)
     ap.add_argument("--mixed-sentences", type=int, default=20000)
     ap.add_argument("--no-tuned", action="store_true", help="use the default batched configuration")
@@ -247,6 +256,13 @@ def main():
     t0 = time.time()
     prefix = puck.synth_wsj6_cached(seed=6)
     g = puck.Grammar.load_text(prefix, puck.VITERBI)
+    if args.only:
+        which = tuple(w for w in args.only.split(",") if w)
+        bad = [w for w in which if w not in SIDE]
+        if bad:
+            sys.exit(f"unknown workloads {bad}; choose from {SIDE}")
+        print(json.dumps(side_workloads(puck, g, prefix, args, batched_config(args), which)))
+        return
     words, offsets = g.sample(seed=args.seed, n=args.sentences, mode=1, min_len=1, max_len=40)
     setup_s = time.time() - t0
     lens = np.diff(offsets)
