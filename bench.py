@@ -170,7 +170,7 @@ def serve(puck, layers, pool, rate, seconds, seed):
             "mean_batch": round(st["mean_batch"], 1)}
 
 
-SIDE = ("c2f_viterbi", "c2f_mbr", "mixed_lengths", "serving")
+SIDE = ("c2f_viterbi", "c2f_mbr", "c2f_reproject", "mixed_lengths", "serving")
 
 
 def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
@@ -188,13 +188,23 @@ def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
             puck, [g1, g6r], words, offsets, 2, cfg,
             f"wsj_1 -> synthetic wsj_6-shape Real inside/outside + MBR, {args.c2f_sentences} PCFG sentences 1-40")
         del g6r
+    if "c2f_reproject" in which:
+        # --reproject: masks keep one bit per wsj_1 substate; fine symbols project through the
+        # wsj_6 split hierarchy.
+        rc, rf = puck.Grammar.reproject(g1, g6v, coarse_level=1, fine_level=6)
+        out["c2f_reproject"] = plain_vs_batched(
+            puck, [rc, rf], words, offsets, 2, cfg,
+            f"wsj_1 -> synthetic wsj_6-shape Viterbi, reprojected through the wsj_6 hierarchy "
+            f"({rc.info.num_coarse} coarse substates), masks at -9, {args.c2f_sentences} PCFG sentences 1-40")
+        del rc, rf
     if "mixed_lengths" in which:
         mw, mo = g6v.sample(seed=4, n=args.mixed_sentences, mode=2, min_len=1, max_len=80)
         out["mixed_lengths"] = plain_vs_batched(
             puck, [g1, g6v], mw, mo, 1, cfg,
             f"c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
     if "serving" in which:
-        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
+        # Request lengths as in the mixed workload: 70% log-normal 1-40, 30% uniform 41-80.
+        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=2, min_len=1, max_len=80)
         pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
         out["serving"] = []
         for rate in (200, 1000, 5000):

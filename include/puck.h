@@ -79,6 +79,16 @@ int32_t puck_abi_version(void);
 /* ---- grammars (CLParserData.make over SimpleGrammar.parseBerkeleyText(prefix, -12, ...)) ---- */
 int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out);
 void puck_grammar_free(puck_grammar_t* g);
+/* CLParser.reprojectGrammar + readHierarchy (CLParser.scala:1153-1208), the --reproject option of
+ * CompileGrammar and CLParser.main (CompileGrammar.scala:57-59, CLParser.scala:1027-1029): new
+ * handles for a coarse/fine pair whose coarse symbols are the coarse grammar's refined symbols;
+ * each fine symbol projects to its ancestor at the coarse split level in the hierarchy, and fine
+ * rules whose projection is not a coarse rule are scored fromLogSpace(-12). hierarchy_path NULL:
+ * the fine prefix + ".hierarchy"; a level < 0: the last number in that grammar's prefix. The inputs
+ * are not modified. */
+int32_t puck_grammar_reproject(const puck_grammar_t* coarse, const puck_grammar_t* fine, const char* hierarchy_path,
+                               int32_t coarse_level, int32_t fine_level, puck_grammar_t** coarse_out,
+                               puck_grammar_t** fine_out);
 /* CLParserData.writeSequence / readSequence (CLParser.scala:1249-1317) with ZipUtil
  * (ZipUtil.scala:13-98): the compiled layers in one zip ("grammar.grz") with the reference's entry
  * names (parserNames, parser-i/grammar, parser-i/structure, parser-i/semiring) and binary table
@@ -109,7 +119,7 @@ int32_t puck_grammar_lexicon(const puck_grammar_t* g, int32_t word, int32_t* tag
                              int32_t cap);
 
 /* ---- seeded inputs ---- */
-/* Synthetic grammar with the wsj_6 shape; writes out_prefix.{numstates,binary,unary,lexicon}. */
+/* Synthetic grammar with the wsj_6 shape; writes out_prefix.{numstates,binary,unary,lexicon,hierarchy}. */
 int32_t puck_synth_wsj6(const char* data_dir, const char* out_prefix, uint64_t seed);
 /* PCFG sentence sampling; mode 0 range filter, 1 log-normal lengths, 2 mixed long tail.
  * words_out needs capacity words_cap; offsets_out needs n+1 entries. */
@@ -122,8 +132,11 @@ int32_t puck_sample_sentences(const puck_grammar_t* g, uint64_t seed, int32_t n,
 enum puck_parser_flags {
   PUCK_FLAG_ONE_ITEM_PER_LANE = 1, /* rule kernels compute one (cell, split) item per lane, not two */
   PUCK_FLAG_PERSISTENT = 2,        /* rule families of a span share one persistent work-queue launch */
-  PUCK_FLAG_GRAPHS = 4             /* parse calls capture each batch shape's device work into a HIP
+  PUCK_FLAG_GRAPHS = 4,            /* parse calls capture each batch shape's device work into a HIP
                                       graph on its second sighting and replay it afterwards */
+  PUCK_FLAG_NO_PLAN_AHEAD = 8      /* masked passes plan each group on the parser's one stream instead
+                                      of ahead on a second one: for several parsers working side by
+                                      side in one process, whose streams may share hardware queues */
 };
 /* new CLParser(data, maxAllocSize) (CLParser.scala:46-48): mem_bytes bounds the device workspace,
  * as the reference's --mem option bounds WorkSpace.allocate (WorkSpace.scala:91-120). */

@@ -10,6 +10,7 @@ the checker. It restates, independently of the product's C++ grammar compiler:
                                                              JavaFriendlyGenRuleMultiply.scala:248-256,
                                                              CLParser.scala:783-785)
   - Viterbi decoder rule arrays                             (CLViterbi.scala:287-309,610-628)
+  - hierarchical reprojection of a coarse/fine pair         (CLParser.scala:1153-1208)
 Symbol ids follow (coarse id in .numstates order, substate); rule order is file order; rules with
 ln p < -12 are dropped (parseBerkeleyText(prefix, -12, ...), CLParser.scala:1026). These are the
 documented choices for behaviour that lives in epic 0.2, which is not in the reference checkout.
@@ -216,3 +217,135 @@ class RefGrammar:
         self.lex_tag = np.array(tags, np.int32)
         self.lex_logp = np.array(logp, np.float64)
         self._lexicon_loaded = True
+
+
+# -- hierarchical reprojection -------------------------------------------------------------------
+def _hier_parse(text):
+    """One .hierarchy tree as nested [label, children, words] lists (PennTreeReader.parseHard as
+    readHierarchy uses it, CLParser.scala:1187-1188): brackets holding only words are a leaf."""
+    toks = text.replace("(", " ( ").replace(")", " ) ").split()
+    pos = 0
+
+    def node():
+        nonlocal pos
+        assert toks[pos] == "(", text
+        label = toks[pos + 1]
+        pos += 2
+        kids, words = [], []
+        while toks[pos] != ")":
+            if toks[pos] == "(":
+                kids.append(node())
+            else:
+                words.append(toks[pos])
+                pos += 1
+        pos += 1
+        if kids and words:
+            raise ValueError("hierarchy node mixes words and subtrees: " + text)
+        return [label, kids, words]
+
+    root = node()
+    if pos != len(toks):
+        raise ValueError("trailing text after hierarchy tree: " + text)
+    return root
+
+
+def _left_height(t):
+    return 0 if not t[1] else 1 + _left_height(t[1][0])
+
+
+def _preorder(t):
+    yield t
+    for k in t[1]:
+        yield from _preorder(k)
+
+
+def _leaves(t):
+    """Words under t, left to right (the node's span of the line's words)."""
+    if not t[1]:
+        return list(t[2])
+    out = []
+    for k in t[1]:
+        out.extend(_leaves(k))
+    return out
+
+
+def read_hierarchy(path, coarse_level, fine_level):
+    """readHierarchy (CLParser.scala:1182-1208): {coarse-level symbol: [fine symbols]}."""
+    sym_map = {}
+    for line in open(path):
+        if not line.strip():
+            continue
+        sym, rest = line.split(None, 1)
+        tree = _hier_parse(rest)
+        total = _left_height(tree)
+        for sub in _preorder(tree):
+            if _left_height(sub) != total - coarse_level:
+                continue
+            if fine_level - 1 == total:
+                fines = [f"{sym}_{q}" for q in _leaves(sub)]
+            else:
+                fines = [f"{sym}_{d[0]}" for d in _preorder(sub) if _left_height(d) == total - fine_level]
+            sym_map[f"{sym}_{sub[0]}"] = fines
+    return sym_map
+
+
+def level_of(name):
+    """The last number in a grammar name (readHierarchy's regex on the grammar names)."""
+    import re
+    m = re.match(r"^.*(?:^|[^0-9])([0-9]+)([^0-9]*)$", name)
+    if not m:
+        raise ValueError("no split level in " + name)
+    return int(m.group(1))
+
+
+def reproject(coarse, fine, hierarchy=None, coarse_level=None, fine_level=None):
+    """reprojectGrammar (CLParser.scala:1153-1180) on two RefGrammars, in place: the coarse
+    grammar's refined symbols become the coarse symbols of both; the coarse layer projects each
+    symbol to itself, the fine layer to its ancestor at the coarse level; fine rules whose
+    projection is not a coarse rule are scored fromLogSpace(-12) (CLParser.scala:1266-1274)."""
+    hierarchy = hierarchy or fine.prefix + ".hierarchy"
+    cl = level_of(coarse.prefix) if coarse_level is None else coarse_level
+    fl = level_of(fine.prefix) if fine_level is None else fine_level
+    up = {}
+    for c, fines in read_hierarchy(hierarchy, cl, fl).items():
+        for f in fines:
+            if up.get(f, c) != c:
+                raise ValueError(f"{f} is under two coarse symbols")
+            up[f] = c
+    up["ROOT_0"] = "ROOT_0"
+    key = lambda s: (coarse.coarse_id[_split(s)[0]], _split(s)[1])
+    names = sorted(coarse.nonterms + coarse.terms, key=key)
+    cid = {s: i for i, s in enumerate(names)}
+    for g in (coarse, fine):
+        g.coarse = list(names)
+        g.coarse_id = None  # base-symbol ids no longer index the coarse symbols
+        g.mask_words = (len(names) + 31) // 32
+    coarse.proj_n = np.array([cid[s] for s in coarse.nonterms], np.int32)
+    coarse.proj_t = np.array([cid[s] for s in coarse.terms], np.int32)
+    fine.proj_n = np.array([cid[up[s]] for s in fine.nonterms], np.int32)
+    fine.proj_t = np.array([cid[up[s]] for s in fine.terms], np.int32)
+
+    def proj(g, fam, which, idx):
+        term = (fam in (2, 3)) if which == "l" else (fam in (1, 3))
+        return (g.proj_t if term else g.proj_n)[idx]
+
+    have_b = set()
+    for fam in range(4):
+        b = coarse.binaries[fam]
+        for p, l, r in zip(coarse.proj_n[b["p"]], proj(coarse, fam, "l", b["l"]), proj(coarse, fam, "r", b["r"])):
+            have_b.add((int(p), int(l), int(r)))
+    have_u = {(int(p), int(c)) for p, c in zip(coarse.proj_n[coarse.unary_n["p"]], coarse.proj_n[coarse.unary_n["c"]])}
+    have_u |= {(int(p), int(c)) for p, c in zip(coarse.proj_n[coarse.unary_t["p"]], coarse.proj_t[coarse.unary_t["c"]])}
+    w12 = np.float32(math.exp(-12.0)) if fine.semiring == "real" else np.float32(-12.0)
+    q12 = java_f6(w12)
+    for fam in range(4):
+        b = fine.binaries[fam]
+        keys = zip(fine.proj_n[b["p"]], proj(fine, fam, "l", b["l"]), proj(fine, fam, "r", b["r"]))
+        miss = np.array([(int(p), int(l), int(r)) not in have_b for p, l, r in keys], bool)
+        b["w"][miss] = w12
+        b["q"][miss] = q12
+    for arr, cp in ((fine.unary_n, fine.proj_n), (fine.unary_t, fine.proj_t)):
+        miss = np.array([(int(p), int(c)) not in have_u for p, c in zip(fine.proj_n[arr["p"]], cp[arr["c"]])], bool)
+        arr["w"][miss] = w12
+        arr["q"][miss] = q12
+    return coarse, fine

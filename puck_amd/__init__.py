@@ -63,6 +63,7 @@ class LaunchConfig(ctypes.Structure):
 FLAG_ONE_ITEM_PER_LANE = 1
 FLAG_PERSISTENT = 2
 FLAG_GRAPHS = 4
+FLAG_NO_PLAN_AHEAD = 8
 
 
 class BatchConfig(ctypes.Structure):
@@ -85,6 +86,8 @@ SIGNATURES = [
     ("puck_abi_version", ctypes.c_int32, []),
     ("puck_grammar_load_text", ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(_P)]),
     ("puck_grammar_free", None, [_P]),
+    ("puck_grammar_reproject", ctypes.c_int32, [_P, _P, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     ("puck_grammar_save_grz", ctypes.c_int32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_char_p]),
     ("puck_grammar_load_grz", ctypes.c_int32, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.c_int32]),
     ("puck_grammar_info", ctypes.c_int32, [_P, ctypes.POINTER(GrammarInfo)]),
@@ -195,6 +198,17 @@ class Grammar:
             out.append(cls(_P(h), f"{path}#{len(out)}", info.semiring))
         return out
 
+    @staticmethod
+    def reproject(coarse, fine, hierarchy=None, coarse_level=-1, fine_level=-1):
+        """CLParser.reprojectGrammar (the --reproject option): new (coarse, fine) layers whose coarse
+        symbols are the coarse grammar's refined symbols, fine symbols projected through the
+        hierarchy (default: fine prefix + ".hierarchy"; levels default to the last number in each
+        prefix)."""
+        hc, hf = _P(), _P()
+        _check(lib().puck_grammar_reproject(coarse._h, fine._h, hierarchy.encode() if hierarchy else None,
+                                            coarse_level, fine_level, ctypes.byref(hc), ctypes.byref(hf)))
+        return Grammar(hc, coarse.prefix, coarse.semiring), Grammar(hf, fine.prefix, fine.semiring)
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.puck_grammar_free(self._h)
@@ -221,7 +235,7 @@ class Grammar:
         return c[:n], s[:n]
 
     def coarse_names(self):
-        """Coarse symbol names in .numstates order."""
+        """Coarse symbol names (.numstates order; a reprojected layer's are coarse substates)."""
         out = []
         buf = ctypes.create_string_buffer(256)
         for c in range(self.info.num_coarse):
@@ -269,7 +283,7 @@ def synth_wsj6_cached(seed=6, cache_dir=None):
     os.makedirs(cache_dir, exist_ok=True)
     prefix = os.path.join(cache_dir, f"wsj6syn_s{seed}.gr")
     done = prefix + ".done"
-    if not os.path.exists(done):
+    if not (os.path.exists(done) and os.path.exists(prefix + ".hierarchy")):
         synth_wsj6(prefix, seed)
         with open(done, "w") as f:
             f.write("ok\n")
@@ -298,12 +312,13 @@ def _parse_with(obj, sentences, vit_fn, mbr_fn):
 class CLParser:
     """Coarse-to-fine CKY parser over grammar layers on one GPU (CLParser.scala:46-952)."""
 
-    def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2, persistent=False, graphs=False):
+    def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2, persistent=False, graphs=False,
+                 plan_ahead=True):
         self.layers = list(layers)
         arr = (_P * len(self.layers))(*[g._h for g in self.layers])
         h = _P()
         flags = ((FLAG_ONE_ITEM_PER_LANE if items_per_lane == 1 else 0) | (FLAG_PERSISTENT if persistent else 0)
-                 | (FLAG_GRAPHS if graphs else 0))
+                 | (FLAG_GRAPHS if graphs else 0) | (0 if plan_ahead else FLAG_NO_PLAN_AHEAD))
         _check(lib().puck_parser_create(arr, len(self.layers), device, mem_bytes, flags, ctypes.byref(h)))
         self._h = h
 

@@ -126,6 +126,9 @@ def cmd_compile(args):
         fine = puck.REAL
     layers = [puck.Grammar.load_text(p, puck.VITERBI) for p in prefixes[:-1]]
     layers.append(puck.Grammar.load_text(prefixes[-1], fine))
+    if args.reproject.lower() == "true" and len(layers) > 1:
+        # CompileGrammar.scala:57-59: the first and last grammars, reprojected, replace the list.
+        layers = list(puck.Grammar.reproject(layers[0], layers[-1]))
     puck.save_grz(args.grammar, layers)
     print(f"wrote {args.grammar}: {len(layers)} layer(s)", file=sys.stderr)
 
@@ -172,6 +175,7 @@ def main(argv=None):
     c.add_argument("--grammar", default="grammar.grz")
     c.add_argument("--viterbi", default="true")
     c.add_argument("--logsum", default="false")
+    c.add_argument("--reproject", default="false")
     c.add_argument("--device", default="0")
     p = sub.add_parser("parse", help="parse text with a grammar.grz (RunParser)")
     p.add_argument("--grammar", default="grammar.grz")

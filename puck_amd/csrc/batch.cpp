@@ -28,6 +28,7 @@ BatchParser::BatchParser(std::vector<std::shared_ptr<Grammar>> layers, int32_t d
     Parser& p = *parsers_.back();
     p.timing = false;
     p.set_persistent(cfg_.persistent);
+    p.set_plan_ahead(false);  // several parsers share the device: one stream each
     p.set_launch_shape(cfg_.piece_target, 1);
     p.use_graphs = cfg_.graphs;
   }

@@ -96,6 +96,27 @@ int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_gramma
 
 void puck_grammar_free(puck_grammar_t* g) { delete g; }
 
+int32_t puck_grammar_reproject(const puck_grammar_t* coarse, const puck_grammar_t* fine, const char* hierarchy_path,
+                               int32_t coarse_level, int32_t fine_level, puck_grammar_t** coarse_out,
+                               puck_grammar_t** fine_out) {
+  return guarded([&]() -> int32_t {
+    if (!coarse || !fine || !coarse_out || !fine_out) return set_error(PUCK_ERR_ARG, "null argument");
+    auto c = std::make_shared<puck::Grammar>(*coarse->g);
+    auto f = std::make_shared<puck::Grammar>(*fine->g);
+    const std::string path = hierarchy_path ? std::string(hierarchy_path) : f->prefix + ".hierarchy";
+    if (coarse_level < 0) coarse_level = puck::grammar_level_from_name(c->prefix);
+    if (fine_level < 0) fine_level = puck::grammar_level_from_name(f->prefix);
+    puck::reproject_grammars(*c, *f, path, coarse_level, fine_level);
+    auto* hc = new puck_grammar;
+    hc->g = c;
+    auto* hf = new puck_grammar;
+    hf->g = f;
+    *coarse_out = hc;
+    *fine_out = hf;
+    return PUCK_OK;
+  });
+}
+
 int32_t puck_grammar_save_grz(const puck_grammar_t* const* layers, int32_t n_layers, const char* path) {
   return guarded([&]() -> int32_t {
     if (!layers || n_layers < 1 || !path) return set_error(PUCK_ERR_ARG, "bad argument");
@@ -277,6 +298,7 @@ int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers
       h->p.reset(new puck::Parser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, ipl));
       h->p->set_persistent((flags & PUCK_FLAG_PERSISTENT) != 0);
       h->p->use_graphs = (flags & PUCK_FLAG_GRAPHS) != 0;
+      h->p->set_plan_ahead((flags & PUCK_FLAG_NO_PLAN_AHEAD) == 0);
     } catch (...) {
       delete h;
       throw;

@@ -196,6 +196,16 @@ class Parser {
     persistent = on;
     drop_graphs();
   }
+  // Masked passes plan every span up front on a second stream (created on first use), so that
+  // plan kernels of later spans run beside earlier spans' rule launches. The two streams wait on
+  // each other (plans read the masks; rule launches read the plans), so this is for a parser that
+  // has the device to itself: parsers running side by side in one process can outnumber the
+  // process's hardware queues, and two parsers whose streams share queues can then block each
+  // other's waits in a cycle. Off, each masked group plans on the engine stream before its launch.
+  void set_plan_ahead(bool on) {
+    plan_ahead_ = on;
+    drop_graphs();
+  }
   // Drops every captured graph (their launches no longer match the configuration).
   void drop_graphs();
   int32_t piece_target_ = 512;
@@ -222,6 +232,7 @@ class Parser {
   int32_t device_;
   int64_t mem_bytes_;
   hipStream_t stream_ = nullptr;
+  bool plan_ahead_ = true;
   hipStream_t plan_stream_ = nullptr;  // mask plans of a masked pass, ahead of its rule launches
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // apps_slot >= 0: the launch's rule applications add the plan counts in

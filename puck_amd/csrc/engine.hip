@@ -910,11 +910,9 @@ __global__ void k_masks(uint32_t* masks_out, const float* inside, const float* o
   const float cutoff = root_score + thresh;
   const bool term = (cell - first) < n;
   const int32_t* proj = term ? proj_t : proj_n;
-  const int nproj = term ? 0 : 0;
-  (void)nproj;
   const float* in = inside + (size_t)cell * cell_size;
   const float* out = outside + (size_t)cell * cell_size;
-  uint32_t m[4] = {0, 0, 0, 0};
+  uint32_t m[kMaxCoarseSymbols / 32] = {};
   for (int sym = lane; sym < num_syms; sym += 64) {
     const float score = in[sym] + out[sym];
     if (score >= cutoff) {
@@ -958,7 +956,7 @@ __global__ void k_mbr(MbrCell* out, const float* inside, const float* outside,
   const int32_t* proj = (cell - first) < n ? proj_t : proj_n;
   const float* in = inside + (size_t)cell * cell_size;
   const float* o = outside + (size_t)cell * cell_size;
-  float margs[128];
+  float margs[kMaxCoarseSymbols];
   for (int c = 0; c < num_coarse; ++c) margs[c] = 0.0f;
   for (int sym = 0; sym < num_syms; ++sym) {
     // CLMBRKernels.scala:145-150: exp(in - root + out) under LogSum, in / root * out otherwise.
@@ -1205,7 +1203,8 @@ T* Layer::upload(const std::vector<T>& v) {
 Layer::Layer(Grammar gr, bool final_l, bool masked_l, int32_t items_per_lane)
     : g(std::move(gr)), final_layer(final_l), masked(masked_l), ipl(items_per_lane) {
   if (ipl != 1 && ipl != 2) throw std::runtime_error("items per lane must be 1 or 2");
-  if (g.mask_words > 4) throw std::runtime_error("at most 128 coarse symbols are supported");
+  if ((int32_t)g.coarse_names.size() > kMaxCoarseSymbols)
+    throw std::runtime_error("at most " + std::to_string(kMaxCoarseSymbols) + " coarse symbols are supported");
   for (int f = 0; f < kNumFamilies; ++f) build_family(f);
   build_reduce(kRInBin, {kFInNN, kFInNT, kFInTN}, g.num_nonterms());
   build_reduce(kRInBin2, {kFInNN, kFInNT, kFInTN, kFInTT}, g.num_nonterms());
@@ -1589,7 +1588,6 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   // Non-blocking: work of other parsers' streams (and of the null stream) never orders against
   // this one, so parsers on one device run concurrently.
   HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  HIP_CHECK(hipStreamCreateWithFlags(&plan_stream_, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreate(&ev0_));
   HIP_CHECK(hipEventCreate(&ev1_));
   // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
@@ -2093,6 +2091,7 @@ void Parser::plan_pass(Batch& B, int32_t li, const std::vector<std::pair<int32_t
     for (auto [k0, kc] : slices[k]) total += plan_ints(B, li, rid, span, k0, kc);
   }
   ensure(mplan_, std::max<size_t>(total, 1) * 4);
+  if (plan_stream_ == nullptr) HIP_CHECK(hipStreamCreateWithFlags(&plan_stream_, hipStreamNonBlocking));
   hipEvent_t fork = next_event();
   HIP_CHECK(hipEventRecord(fork, stream_));
   HIP_CHECK(hipStreamWaitEvent(plan_stream_, fork, 0));
@@ -2276,8 +2275,8 @@ void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
 // on the plan stream (plan_pass), and the plan stream is joined back at the end.
 void Parser::run_groups(Batch& B, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
                         bool use_masks) {
-  if (!use_masks) {
-    for (auto [rid, w] : groups) run_group(B, li, rid, w, false, nullptr);
+  if (!use_masks || !plan_ahead_) {
+    for (auto [rid, w] : groups) run_group(B, li, rid, w, use_masks, nullptr);
     return;
   }
   std::vector<std::vector<PlannedSlice>> plans;

@@ -2,6 +2,7 @@
 #include "grammar.h"
 
 #include <algorithm>
+#include <cctype>
 #include <charconv>
 #include <cmath>
 #include <cstdio>
@@ -152,6 +153,77 @@ BlockPartition coarse_parent_partition(const std::vector<int32_t>& coarse_key_pe
 }
 
 // ---------------------------------------------------------------------------------------------
+// Tables derived from the rules and the symbol projections: CoarseParent partitions and the
+// Viterbi decoder arrays. Rebuilt after a reprojection changes the projections or rule scores.
+void build_tables(Grammar& g) {
+  const int32_t num_coarse = (int32_t)g.coarse_names.size();
+  g.mask_words = (num_coarse + 31) / 32;
+  g.dec_offsets.clear();
+  g.dec_scores.clear();
+  g.dec_left.clear();
+  g.dec_right.clear();
+  // CoarseParent partitions for the 4 inside families and the 8 rotated outside families
+  // (CLOutside.scala:61-71 rotates the updated child into the parent slot).
+  for (int f = 0; f < 4; ++f) {
+    const bool lt = (f == kTN || f == kTT), rt = (f == kNT || f == kTT);
+    std::vector<int32_t> kp, kl, kr;
+    for (auto& r : g.binaries[f]) {
+      kp.push_back(g.proj_n[r.p]);
+      kl.push_back(lt ? g.proj_t[r.l] : g.proj_n[r.l]);
+      kr.push_back(rt ? g.proj_t[r.r] : g.proj_n[r.r]);
+    }
+    g.inside_blocks[f] = coarse_parent_partition(kp, num_coarse, g.mask_words);
+    g.outside_l_blocks[f] = coarse_parent_partition(kl, num_coarse, g.mask_words);
+    g.outside_r_blocks[f] = coarse_parent_partition(kr, num_coarse, g.mask_words);
+  }
+
+  // Viterbi decoder tables (CLViterbi.scala:287-309, 610-628).
+  {
+    const int32_t nn = g.num_nonterms();
+    g.dec_offsets.assign(1, 0);
+    auto add_group = [&](std::vector<int32_t> parents, std::vector<float> scores,
+                         std::vector<int32_t> lefts, std::vector<int32_t> rights, bool binary) {
+      std::vector<int32_t> order(parents.size());
+      for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int32_t a, int32_t b) { return parents[a] < parents[b]; });
+      const int32_t start = (int32_t)g.dec_scores.size();
+      std::vector<int32_t> sorted_parents;
+      for (int32_t i : order) {
+        sorted_parents.push_back(parents[i]);
+        g.dec_scores.push_back(scores[i]);
+        g.dec_left.push_back(lefts[i]);
+        if (binary) g.dec_right.push_back(rights[i]);
+      }
+      for (int32_t sym = 0; sym < nn; ++sym) {
+        auto it = std::upper_bound(sorted_parents.begin(), sorted_parents.end(), sym);
+        g.dec_offsets.push_back(start + (int32_t)(it - sorted_parents.begin()));
+      }
+    };
+    for (int f = 0; f < 4; ++f) {
+      std::vector<int32_t> p, l, r;
+      std::vector<float> s;
+      for (auto& br : g.binaries[f]) {
+        p.push_back(br.p);
+        l.push_back(br.l);
+        r.push_back(br.r);
+        s.push_back(br.w);
+      }
+      add_group(p, s, l, r, true);
+    }
+    for (auto* us : {&g.unary_n, &g.unary_t}) {
+      std::vector<int32_t> p, c;
+      std::vector<float> s;
+      for (auto& ur : *us) {
+        p.push_back(ur.p);
+        c.push_back(ur.c);
+        s.push_back(ur.w);
+      }
+      add_group(p, s, c, {}, false);
+    }
+  }
+}
+
 Grammar load_text_grammar(const std::string& prefix, Semiring semiring, double log_threshold) {
   Grammar g;
   g.prefix = prefix;
@@ -298,66 +370,7 @@ Grammar load_text_grammar(const std::string& prefix, Semiring semiring, double l
     }
   }
 
-  // CoarseParent partitions for the 4 inside families and the 8 rotated outside families
-  // (CLOutside.scala:61-71 rotates the updated child into the parent slot).
-  for (int f = 0; f < 4; ++f) {
-    const bool lt = (f == kTN || f == kTT), rt = (f == kNT || f == kTT);
-    std::vector<int32_t> kp, kl, kr;
-    for (auto& r : g.binaries[f]) {
-      kp.push_back(g.proj_n[r.p]);
-      kl.push_back(lt ? g.proj_t[r.l] : g.proj_n[r.l]);
-      kr.push_back(rt ? g.proj_t[r.r] : g.proj_n[r.r]);
-    }
-    g.inside_blocks[f] = coarse_parent_partition(kp, num_coarse, g.mask_words);
-    g.outside_l_blocks[f] = coarse_parent_partition(kl, num_coarse, g.mask_words);
-    g.outside_r_blocks[f] = coarse_parent_partition(kr, num_coarse, g.mask_words);
-  }
-
-  // Viterbi decoder tables (CLViterbi.scala:287-309, 610-628).
-  {
-    const int32_t nn = g.num_nonterms();
-    g.dec_offsets.assign(1, 0);
-    auto add_group = [&](std::vector<int32_t> parents, std::vector<float> scores,
-                         std::vector<int32_t> lefts, std::vector<int32_t> rights, bool binary) {
-      std::vector<int32_t> order(parents.size());
-      for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int32_t a, int32_t b) { return parents[a] < parents[b]; });
-      const int32_t start = (int32_t)g.dec_scores.size();
-      std::vector<int32_t> sorted_parents;
-      for (int32_t i : order) {
-        sorted_parents.push_back(parents[i]);
-        g.dec_scores.push_back(scores[i]);
-        g.dec_left.push_back(lefts[i]);
-        if (binary) g.dec_right.push_back(rights[i]);
-      }
-      for (int32_t sym = 0; sym < nn; ++sym) {
-        auto it = std::upper_bound(sorted_parents.begin(), sorted_parents.end(), sym);
-        g.dec_offsets.push_back(start + (int32_t)(it - sorted_parents.begin()));
-      }
-    };
-    for (int f = 0; f < 4; ++f) {
-      std::vector<int32_t> p, l, r;
-      std::vector<float> s;
-      for (auto& br : g.binaries[f]) {
-        p.push_back(br.p);
-        l.push_back(br.l);
-        r.push_back(br.r);
-        s.push_back(br.w);
-      }
-      add_group(p, s, l, r, true);
-    }
-    for (auto* us : {&g.unary_n, &g.unary_t}) {
-      std::vector<int32_t> p, c;
-      std::vector<float> s;
-      for (auto& ur : *us) {
-        p.push_back(ur.p);
-        c.push_back(ur.c);
-        s.push_back(ur.w);
-      }
-      add_group(p, s, c, {}, false);
-    }
-  }
+  build_tables(g);
 
   // Lexicon: "TAG word [p_0, ..., p_k]" = P(word | TAG_i).
   {
@@ -414,6 +427,222 @@ Grammar load_text_grammar(const std::string& prefix, Semiring semiring, double l
     }
   }
   return g;
+}
+
+namespace {
+
+// One bracketed node of a .hierarchy line, read the way readHierarchy reads it with
+// PennTreeReader.parseHard (CLParser.scala:1187-1188): a node whose brackets hold only words is a
+// leaf spanning those words; other nodes hold only subtrees.
+struct HierNode {
+  std::string label;
+  std::vector<int32_t> kids;
+  int32_t w0 = 0, w1 = 0;  // the node's words are [w0, w1) of the line's word list
+  bool has_words = false;
+};
+
+struct HierTree {
+  std::vector<HierNode> nodes;  // preorder, nodes[0] is the root
+  std::vector<std::string> words;
+};
+
+HierTree parse_hier_tree(const std::string& text) {
+  HierTree t;
+  size_t i = 0;
+  auto skip = [&]() {
+    while (i < text.size() && std::isspace((unsigned char)text[i])) ++i;
+  };
+  auto atom = [&]() {
+    skip();
+    size_t j = i;
+    while (j < text.size() && !std::isspace((unsigned char)text[j]) && text[j] != '(' && text[j] != ')') ++j;
+    if (j == i) fail("bad hierarchy tree: " + text);
+    std::string a = text.substr(i, j - i);
+    i = j;
+    return a;
+  };
+  std::vector<int32_t> open;
+  skip();
+  if (i >= text.size() || text[i] != '(') fail("hierarchy tree must start with '(': " + text);
+  do {
+    skip();
+    if (i >= text.size()) fail("unbalanced hierarchy tree: " + text);
+    if (text[i] == '(') {
+      ++i;
+      HierNode nd;
+      nd.label = atom();
+      nd.w0 = (int32_t)t.words.size();
+      const int32_t id = (int32_t)t.nodes.size();
+      if (!open.empty()) {
+        if (t.nodes[open.back()].has_words) fail("hierarchy node mixes words and subtrees: " + text);
+        t.nodes[open.back()].kids.push_back(id);
+      }
+      t.nodes.push_back(std::move(nd));
+      open.push_back(id);
+    } else if (text[i] == ')') {
+      ++i;
+      t.nodes[open.back()].w1 = (int32_t)t.words.size();
+      open.pop_back();
+    } else {
+      HierNode& nd = t.nodes[open.back()];
+      if (!nd.kids.empty()) fail("hierarchy node mixes words and subtrees: " + text);
+      nd.has_words = true;
+      t.words.push_back(atom());
+    }
+  } while (!open.empty());
+  skip();
+  if (i != text.size()) fail("trailing text after hierarchy tree: " + text);
+  return t;
+}
+
+}  // namespace
+
+std::map<std::string, std::vector<std::string>> read_hierarchy(const std::string& path, int32_t coarse_level,
+                                                               int32_t fine_level) {
+  std::map<std::string, std::vector<std::string>> out;
+  auto in = open_or_fail(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    size_t a = 0;
+    while (a < line.size() && std::isspace((unsigned char)line[a])) ++a;
+    if (a == line.size()) continue;
+    size_t b = a;
+    while (b < line.size() && !std::isspace((unsigned char)line[b])) ++b;
+    const std::string sym = line.substr(a, b - a);
+    const HierTree t = parse_hier_tree(line.substr(b));
+    const int32_t nn = (int32_t)t.nodes.size();
+    // leftHeight: 0 for a leaf, else 1 + leftHeight of the first child. Children follow their
+    // parent in preorder, so one backward sweep fills heights and subtree sizes.
+    std::vector<int32_t> lh(nn, 0), size(nn, 1);
+    for (int32_t v = nn - 1; v >= 0; --v) {
+      const HierNode& nd = t.nodes[v];
+      if (!nd.kids.empty()) lh[v] = 1 + lh[nd.kids[0]];
+      for (int32_t k : nd.kids) size[v] += size[k];
+    }
+    const int32_t total = lh[0];
+    for (int32_t v = 0; v < nn; ++v) {
+      if (lh[v] != total - coarse_level) continue;
+      std::vector<std::string> fines;
+      if (fine_level - 1 == total) {
+        for (int32_t w = t.nodes[v].w0; w < t.nodes[v].w1; ++w) fines.push_back(sym + "_" + t.words[w]);
+      } else {
+        for (int32_t d = v; d < v + size[v]; ++d)
+          if (lh[d] == total - fine_level) fines.push_back(sym + "_" + t.nodes[d].label);
+      }
+      out[sym + "_" + t.nodes[v].label] = std::move(fines);  // a later node of one name replaces it
+    }
+  }
+  return out;
+}
+
+int32_t grammar_level_from_name(const std::string& name) {
+  // readHierarchy's "^.*(?:^|[^0-9])([0-9]+)([^0-9]*)$" -> the last run of digits.
+  size_t end = name.size();
+  while (end > 0 && !std::isdigit((unsigned char)name[end - 1])) --end;
+  if (end == 0) fail("no split level in grammar name: " + name);
+  size_t beg = end;
+  while (beg > 0 && std::isdigit((unsigned char)name[beg - 1])) --beg;
+  return std::stoi(name.substr(beg, end - beg));
+}
+
+void reproject_grammars(Grammar& coarse, Grammar& fine, const std::string& hierarchy_path, int32_t coarse_level,
+                        int32_t fine_level) {
+  if (coarse_level < 0 || fine_level <= coarse_level)
+    fail("reprojection needs 0 <= coarse level < fine level (got " + std::to_string(coarse_level) + ", " +
+         std::to_string(fine_level) + ")");
+  // fine refined symbol -> coarse-level symbol (reverseSymMap). The root has no hierarchy line;
+  // the reference maps its root label to itself (TOP_0 there, ROOT_0 in the text files).
+  std::unordered_map<std::string, std::string> up;
+  for (const auto& kv : read_hierarchy(hierarchy_path, coarse_level, fine_level))
+    for (const auto& f : kv.second) {
+      auto ins = up.emplace(f, kv.first);
+      if (!ins.second && ins.first->second != kv.first)
+        fail("hierarchy puts " + f + " under both " + ins.first->second + " and " + kv.first);
+    }
+  up["ROOT_0"] = "ROOT_0";
+
+  // The coarse layer's refined symbols become the coarse symbols of both layers (the label index
+  // of coarseGrammar.topology), in (base symbol, substate) order.
+  std::vector<const Symbol*> cs;
+  for (const auto& x : coarse.nonterms) cs.push_back(&x);
+  for (const auto& x : coarse.terms) cs.push_back(&x);
+  std::sort(cs.begin(), cs.end(), [](const Symbol* a, const Symbol* b) {
+    return a->coarse != b->coarse ? a->coarse < b->coarse : a->sub < b->sub;
+  });
+  if ((int32_t)cs.size() > kMaxCoarseSymbols)
+    fail("reprojection gives " + std::to_string(cs.size()) + " coarse symbols; at most " +
+         std::to_string(kMaxCoarseSymbols) + " are supported");
+  std::vector<std::string> names;
+  std::unordered_map<std::string, int32_t> cid;
+  for (const Symbol* x : cs) {
+    cid[x->name] = (int32_t)names.size();
+    names.push_back(x->name);
+  }
+
+  // Coarse layer: identity refinements (GrammarRefinements.identity), scores unchanged.
+  for (size_t i = 0; i < coarse.nonterms.size(); ++i)
+    coarse.proj_n[i] = coarse.nonterms[i].coarse = cid.at(coarse.nonterms[i].name);
+  for (size_t i = 0; i < coarse.terms.size(); ++i)
+    coarse.proj_t[i] = coarse.terms[i].coarse = cid.at(coarse.terms[i].name);
+  coarse.coarse_names = names;
+  coarse.coarse_substates.assign(names.size(), 1);
+
+  // Rules of the coarse layer over the new symbol ids: a fine rule keeps its score only if its
+  // projection is one of them (GrammarRefinements(..., skipMissingCoarseRules = true) leaves the
+  // others unprojected, and CLParserData.make scores those fromLogSpace(-12), CLParser.scala:1266-1274).
+  auto key3 = [](int64_t p, int64_t l, int64_t r) { return (p << 40) | (l << 20) | r; };
+  std::unordered_set<int64_t> cbin, cun;
+  for (int f = 0; f < 4; ++f) {
+    const bool lt = (f == kTN || f == kTT), rt = (f == kNT || f == kTT);
+    for (const auto& r : coarse.binaries[f])
+      cbin.insert(key3(coarse.proj_n[r.p], lt ? coarse.proj_t[r.l] : coarse.proj_n[r.l],
+                       rt ? coarse.proj_t[r.r] : coarse.proj_n[r.r]));
+  }
+  for (const auto& r : coarse.unary_n) cun.insert(key3(coarse.proj_n[r.p], coarse.proj_n[r.c], 0));
+  for (const auto& r : coarse.unary_t) cun.insert(key3(coarse.proj_n[r.p], coarse.proj_t[r.c], 0));
+
+  // Fine layer: each refined symbol projects to its coarse-level ancestor.
+  auto to_coarse = [&](const std::string& name) {
+    auto it = up.find(name);
+    if (it == up.end()) fail("symbol " + name + " of " + fine.prefix + " is not in " + hierarchy_path);
+    auto jt = cid.find(it->second);
+    if (jt == cid.end()) fail("coarse-level symbol " + it->second + " is not a symbol of " + coarse.prefix);
+    return jt->second;
+  };
+  std::vector<int32_t> per_coarse(names.size(), 0);
+  for (size_t i = 0; i < fine.nonterms.size(); ++i) {
+    fine.proj_n[i] = fine.nonterms[i].coarse = to_coarse(fine.nonterms[i].name);
+    per_coarse[fine.proj_n[i]]++;
+  }
+  for (size_t i = 0; i < fine.terms.size(); ++i) {
+    fine.proj_t[i] = fine.terms[i].coarse = to_coarse(fine.terms[i].name);
+    per_coarse[fine.proj_t[i]]++;
+  }
+  fine.coarse_names = names;
+  fine.coarse_substates = per_coarse;
+  const float w12 = fine.semiring == kReal ? (float)std::exp(-12.0) : -12.0f;
+  const float q12 = java_format_f6(w12);
+  for (int f = 0; f < 4; ++f) {
+    const bool lt = (f == kTN || f == kTT), rt = (f == kNT || f == kTT);
+    for (auto& r : fine.binaries[f])
+      if (!cbin.count(key3(fine.proj_n[r.p], lt ? fine.proj_t[r.l] : fine.proj_n[r.l],
+                           rt ? fine.proj_t[r.r] : fine.proj_n[r.r]))) {
+        r.w = w12;
+        r.q = q12;
+      }
+  }
+  for (auto& r : fine.unary_n)
+    if (!cun.count(key3(fine.proj_n[r.p], fine.proj_n[r.c], 0))) {
+      r.w = w12;
+      r.q = q12;
+    }
+  for (auto& r : fine.unary_t)
+    if (!cun.count(key3(fine.proj_n[r.p], fine.proj_t[r.c], 0))) {
+      r.w = w12;
+      r.q = q12;
+    }
+  build_tables(coarse);
+  build_tables(fine);
 }
 
 }  // namespace puck

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -104,5 +105,27 @@ std::string java_format_f6_string(float v);
 
 BlockPartition coarse_parent_partition(const std::vector<int32_t>& coarse_key_per_rule,
                                        int32_t num_coarse, int32_t mask_words);
+
+// Recomputes mask_words, the CoarseParent partitions and the decoder tables from the rules and the
+// symbol projections.
+void build_tables(Grammar& g);
+
+// Coarse symbols per grammar supported by the mask and MBR kernels (8 mask words).
+constexpr int32_t kMaxCoarseSymbols = 256;
+
+// readHierarchy (CLParser.scala:1182-1208): per coarse-level symbol "SYM_k" of a .hierarchy file,
+// the fine-level symbols below it. Levels count split rounds (0 = unsplit).
+std::map<std::string, std::vector<std::string>> read_hierarchy(const std::string& path, int32_t coarse_level,
+                                                               int32_t fine_level);
+// The split level in a grammar's name: its last run of digits ("wsj_6.gr" -> 6), as readHierarchy
+// derives both levels from the grammar names.
+int32_t grammar_level_from_name(const std::string& name);
+// reprojectGrammar (CLParser.scala:1153-1180), applied in place to a coarse/fine pair: the coarse
+// layer's refined symbols become the coarse symbols of both layers (identity projection in the
+// coarse layer, the hierarchy's ancestor at coarse_level in the fine layer), so masks keep one bit
+// per coarse substate; fine rules whose projection is not a coarse rule are scored
+// fromLogSpace(-12). Throws std::runtime_error on a symbol the hierarchy or the coarse grammar lacks.
+void reproject_grammars(Grammar& coarse, Grammar& fine, const std::string& hierarchy_path, int32_t coarse_level,
+                        int32_t fine_level);
 
 }  // namespace puck

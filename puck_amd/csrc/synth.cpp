@@ -288,6 +288,15 @@ std::string synth_wsj6(const std::string& dir, const std::string& out, uint64_t 
     }
     std::fclose(f);
   }
+  {
+    // The split hierarchy of wsj_6: the synthetic symbols have its substate counts, so its trees
+    // describe them (used by the hierarchical reprojection of a coarse/fine pair).
+    std::ifstream in(dir + "/wsj_6.gr.hierarchy", std::ios::binary);
+    if (!in) fail("missing wsj_6.gr.hierarchy");
+    std::ofstream o(out + ".hierarchy", std::ios::binary);
+    o << in.rdbuf();
+    if (!o) fail("cannot write " + out + ".hierarchy");
+  }
   return summary;
 }
 

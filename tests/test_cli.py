@@ -56,3 +56,14 @@ def test_compile_writes_grz(tmp_path, puck):
               "--logsum", "true"])
     with zipfile.ZipFile(out) as z:
         assert z.read("parser-1/semiring") == b"LogSum"
+
+
+def test_compile_reproject(tmp_path, puck):
+    out = tmp_path / "r.grz"
+    prefix = puck.grammar_path("wsj_0") + ":" + puck.grammar_path("wsj_1")
+    cli.main(["compile", "--textGrammarPrefix", prefix, "--grammar", str(out), "--reproject", "true"])
+    c, f = puck.Grammar.load_grz(str(out))
+    rc, rf = puck.Grammar.reproject(puck.Grammar.load_text(puck.grammar_path("wsj_0")),
+                                    puck.Grammar.load_text(puck.grammar_path("wsj_1")))
+    assert f.coarse_names() == rf.coarse_names() and f.coarse_names()[1] == "S_0"
+    assert np.array_equal(f.rules(0)[1].view(np.int32), rf.rules(0)[1].view(np.int32))

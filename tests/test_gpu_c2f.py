@@ -64,3 +64,10 @@ def test_masked_real_partitions_and_mbr(puck, wsj0, wsj1_real, c2f_sents, oracle
     for i, r in enumerate(ref):
         np.testing.assert_allclose(mbr[i][:, 4].view(np.float32), r["mbr"][:, 4].view(np.float32),
                                    rtol=1e-3, atol=1e-4)
+
+
+def test_planning_on_one_stream_same_trees(puck, wsj0, wsj1, c2f_sents):
+    """Plans made ahead on a second stream and plans made on the engine stream give the same trees."""
+    ahead = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True).parse(c2f_sents)
+    inline = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True, plan_ahead=False).parse(c2f_sents)
+    assert all(np.array_equal(a, b) for a, b in zip(ahead, inline))
