@@ -198,10 +198,10 @@ class Parser {
   }
   // Masked passes plan every span up front on a second stream (created on first use), so that
   // plan kernels of later spans run beside earlier spans' rule launches. The two streams wait on
-  // each other (plans read the masks; rule launches read the plans), so this is for a parser that
-  // has the device to itself: parsers running side by side in one process can outnumber the
-  // process's hardware queues, and two parsers whose streams share queues can then block each
-  // other's waits in a cycle. Off, each masked group plans on the engine stream before its launch.
+  // each other (plans read the masks; rule launches read the plans). This is for a parser that has
+  // the device to itself: three busy parsers doing so side by side in one process (six such
+  // streams over the process's four hardware queues) stalled for good. Off, each masked group
+  // plans on the engine stream right before its launch.
   void set_plan_ahead(bool on) {
     plan_ahead_ = on;
     drop_graphs();
