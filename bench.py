@@ -203,8 +203,8 @@ def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
             puck, [g1, g6v], mw, mo, 1, cfg,
             f"c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
     if "serving" in which:
-        # Request lengths as in the mixed workload: 70% log-normal 1-40, 30% uniform 41-80.
-        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=2, min_len=1, max_len=80)
+        # Request lengths as in the coarse-to-fine workload: PCFG samples of length 1-40.
+        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
         pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
         out["serving"] = []
         for rate in (200, 1000, 5000):
