@@ -203,8 +203,9 @@ class Parser {
   // streams over the process's four hardware queues) stalled for good. Off, each masked group
   // plans on the engine stream right before its launch.
   void set_plan_ahead(bool on) {
+    if (plan_ahead_ == on) return;
     plan_ahead_ = on;
-    drop_graphs();
+    drop_graphs();  // captured graphs hold the other stream layout
   }
   // Drops every captured graph (their launches no longer match the configuration).
   void drop_graphs();
