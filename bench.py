@@ -258,7 +258,7 @@ def main():
         print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 3}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
+        print(f"bench.py measures one GPU only: the engine has no multi-GPU path (asked for {args.gpus}); "
               "sentences are independent and would shard by sentence across GPUs.", file=sys.stderr)
         sys.exit(2)
 
