@@ -2,6 +2,7 @@
 #include "server.h"
 
 #include <algorithm>
+#include <climits>
 #include <stdexcept>
 
 namespace puck {
@@ -13,7 +14,11 @@ Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   mbr_final_ = layers.back()->semiring != kViterbi;
   const int64_t per_slot = mem_bytes / max_inflight;
   int32_t max_cell = 0;
-  for (auto& g : layers) max_cell = std::max(max_cell, g->cell_size);
+  num_words_ = INT32_MAX;
+  for (auto& g : layers) {
+    max_cell = std::max(max_cell, g->cell_size);
+    num_words_ = std::min(num_words_, (int32_t)g->lex_offsets.size() - 1);
+  }
   // Same split as Parser::make_batches: 30% of a slot's memory for two charts per layer.
   max_cells_ = std::max<int64_t>(1, (int64_t)(per_slot * 0.3) / ((int64_t)max_cell * 4));
   for (int32_t i = 0; i < max_inflight; ++i) {
@@ -36,6 +41,9 @@ Server::~Server() {
 
 int64_t Server::submit(const int32_t* words, int32_t len) {
   if (len < 1 || len > 255) throw std::runtime_error("sentence length must be in [1, 255]");
+  // Rejected here, a bad request cannot fail the batch it would have joined.
+  for (int32_t i = 0; i < len; ++i)
+    if (words[i] < 0 || words[i] >= num_words_) throw std::runtime_error("word id out of range of the grammar lexicon");
   std::lock_guard<std::mutex> lk(mu_);
   Request r;
   r.ticket = next_ticket_++;

@@ -62,6 +62,7 @@ class Server {
   std::vector<std::thread> threads_;
   int32_t max_batch_;
   int64_t max_cells_;
+  int32_t num_words_ = 0;  // word ids every layer's lexicon covers: [0, num_words_)
   bool mbr_final_;
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;

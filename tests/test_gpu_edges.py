@@ -61,3 +61,15 @@ def test_rejected_inputs_leave_the_parser_usable(puck, wsj1):
             p.parse((words, offsets))
     after = p.parse((good_w, good_o))
     assert all(np.array_equal(a, b) for a, b in zip(before, after))
+
+
+def test_batch_engine_rejects_bad_input_and_stays_usable(puck, wsj1):
+    words, offsets = wsj1.sample(seed=95, n=30, mode=0, min_len=2, max_len=20)
+    bp = puck.BatchParser([wsj1], mem_bytes=2 << 30, bucket_cells=256)
+    before = bp.parse((words, offsets))
+    bad = words.copy()
+    bad[3] = wsj1.info.num_words
+    with pytest.raises(puck.PuckError, match="word id out of range"):
+        bp.parse((bad, offsets))
+    after = bp.parse((words, offsets))
+    assert all(np.array_equal(a, b) for a, b in zip(before, after))

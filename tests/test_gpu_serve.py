@@ -45,3 +45,23 @@ def test_server_coarse_to_fine(puck, wsj0, wsj1):
     srv.flush()
     for i, s in enumerate(sents):
         assert np.array_equal(srv.result(tickets[i], len(s), wait=False), want[i])
+
+
+def test_server_rejects_bad_requests_at_submit(puck, wsj1):
+    """A request with an unknown word id or a bad length is refused by submit, so it cannot fail the
+    batch it would have joined; the requests around it complete normally."""
+    words, offsets = wsj1.sample(seed=79, n=12, mode=0, min_len=3, max_len=15)
+    want = puck.CLParser([wsj1], mem_bytes=1 << 30).parse((words, offsets))
+    srv = puck.Server([wsj1], mem_bytes=2 << 30, max_inflight=2)
+    sents = split_sentences(words, offsets)
+    tickets = []
+    for i, s in enumerate(sents):
+        tickets.append(srv.submit(s))
+        if i == 5:
+            with pytest.raises(puck.PuckError, match="word id out of range"):
+                srv.submit(np.array([0, wsj1.info.num_words, 1], np.int32))
+            with pytest.raises(puck.PuckError, match="sentence length"):
+                srv.submit(np.zeros(256, np.int32))
+    srv.flush()
+    for i, s in enumerate(sents):
+        assert np.array_equal(srv.result(tickets[i], len(s)), want[i]), f"request {i}"
