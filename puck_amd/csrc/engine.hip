@@ -706,6 +706,9 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
 // through a per-wave LDS buffer with coalesced loads; each lane then folds its own cell's items.
 // The block's results are transposed through LDS so that each cell's targets are written as one
 // contiguous run of the chart row.
+// Partial values a reduce wave keeps in flight within one family's run of a target's entries.
+constexpr int kReduceBatch = 8;
+
 template <int SEMI>
 __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   constexpr int CAP = 512;  // floats per wave buffer
@@ -723,110 +726,141 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   if (t < a.n_t && a.act[0] != nullptr) {
     // Masked (planned) groups: a cell's items are contiguous in the plan's order and few, so each
     // lane reads its own items straight from the partial rows, four slots of a family at a time.
-    const int e_end = a.off[t + 1];
-    for (int e = a.off[t]; e < e_end;) {
-      const int f = a.fam[e];
-      int run = e + 1;
-      while (run < e_end && a.fam[run] == f) ++run;
-      const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
-      const bool use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
-      if (__ballot(use) != 0ull && use) {
-        const int32_t* pre = a.prefix[f];
-        const int pos = a.posof[f] != nullptr ? a.posof[f][my_c] : my_k;
-        const int lo = pre[pos], hi = pre[pos + 1];
-        const size_t pitch = (size_t)a.pitch[f];
-        const float* base = a.partial[f];
-        int q = e;
-        for (; q + 4 <= run; q += 4) {
-          const float* p0 = base + (size_t)a.slot[q] * pitch;
-          const float* p1 = base + (size_t)a.slot[q + 1] * pitch;
-          const float* p2 = base + (size_t)a.slot[q + 2] * pitch;
-          const float* p3 = base + (size_t)a.slot[q + 3] * pitch;
-          for (int j = lo; j < hi; ++j) {
-            const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
-            if (SEMI == kViterbi) {
-              acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
-            } else {
-              acc = semi_add<SEMI>(acc, v0);
-              acc = semi_add<SEMI>(acc, v1);
-              acc = semi_add<SEMI>(acc, v2);
-              acc = semi_add<SEMI>(acc, v3);
+    // The target's entries (family, slot) are read 64 at a time with one vector load each and taken
+    // lane by lane.
+    const int e0 = a.off[t], ne = a.off[t + 1] - e0;
+    for (int b0 = 0; b0 < ne; b0 += 64) {
+      const int cnt = min(64, ne - b0);
+      const int li = e0 + b0 + min(lane, cnt - 1);
+      const int lf = a.fam[li], ls = a.slot[li];
+      for (int k = 0; k < cnt;) {
+        const int f = __builtin_amdgcn_readlane(lf, k);
+        int kr = k + 1;
+        while (kr < cnt && __builtin_amdgcn_readlane(lf, kr) == f) ++kr;
+        const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
+        const bool use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
+        if (__ballot(use) != 0ull && use) {
+          const int32_t* pre = a.prefix[f];
+          const int pos = a.posof[f] != nullptr ? a.posof[f][my_c] : my_k;
+          const int lo = pre[pos], hi = pre[pos + 1];
+          const size_t pitch = (size_t)a.pitch[f];
+          const float* base = a.partial[f];
+          int q = k;
+          for (; q + 4 <= kr; q += 4) {
+            const float* p0 = base + (size_t)__builtin_amdgcn_readlane(ls, q) * pitch;
+            const float* p1 = base + (size_t)__builtin_amdgcn_readlane(ls, q + 1) * pitch;
+            const float* p2 = base + (size_t)__builtin_amdgcn_readlane(ls, q + 2) * pitch;
+            const float* p3 = base + (size_t)__builtin_amdgcn_readlane(ls, q + 3) * pitch;
+            for (int j = lo; j < hi; ++j) {
+              const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
+              if (SEMI == kViterbi) {
+                acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
+              } else {
+                acc = semi_add<SEMI>(acc, v0);
+                acc = semi_add<SEMI>(acc, v1);
+                acc = semi_add<SEMI>(acc, v2);
+                acc = semi_add<SEMI>(acc, v3);
+              }
             }
           }
+          for (; q < kr; ++q) {
+            const float* p0 = base + (size_t)__builtin_amdgcn_readlane(ls, q) * pitch;
+            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, p0[j]);
+          }
         }
-        for (; q < run; ++q) {
-          const float* p0 = base + (size_t)a.slot[q] * pitch;
-          for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, p0[j]);
-        }
+        k = kr;
       }
-      e = run;
     }
   } else if (t < a.n_t) {
-    int cur_f = -1;
-    bool use = true, any_use = true;
-    for (int e = a.off[t]; e < a.off[t + 1]; ++e) {
-      const int f = a.fam[e];
-      if (f != cur_f && a.act[f] != nullptr) {
-        // Entries of a target are grouped by family, and all slots of the target in one family
-        // share its block: test it once per family, and skip the family if no lane needs it.
-        cur_f = f;
-        const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
-        use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
-        any_use = __ballot(use) != 0ull;
-      }
-      if (!any_use) continue;
-      const float* p = a.partial[f] + (size_t)a.slot[e] * a.pitch[f];
-      if (a.prefix[f] == nullptr) {
-        // Split-major items: split j of cell k is item j * num_cells + k; lanes read consecutive
-        // cells of one split row.
-        const int uc = a.uniform_count[f];
-        if (my_k < kend && use) {
-          const float* q = p + my_k;
-          int j = 0;
-          for (; j + 4 <= uc; j += 4) {
-            const float v0 = q[(size_t)j * a.num_cells], v1 = q[(size_t)(j + 1) * a.num_cells];
-            const float v2 = q[(size_t)(j + 2) * a.num_cells], v3 = q[(size_t)(j + 3) * a.num_cells];
+    // The target's entries (family, slot) are read 64 at a time with one vector load each and
+    // taken lane by lane; entries of one family are consecutive, so a family's fields and block
+    // test are read once per run. Within a uniform family's run the (entry, split row) values are
+    // loaded kReduceBatch at a time across entries and folded in entry order.
+    const int e0 = a.off[t], ne = a.off[t + 1] - e0;
+    for (int b0 = 0; b0 < ne; b0 += 64) {
+      const int cnt = min(64, ne - b0);
+      const int li = e0 + b0 + min(lane, cnt - 1);
+      const int lf = a.fam[li], ls = a.slot[li];
+      for (int k = 0; k < cnt;) {
+        const int f = __builtin_amdgcn_readlane(lf, k);
+        int kr = k + 1;
+        while (kr < cnt && __builtin_amdgcn_readlane(lf, kr) == f) ++kr;
+        bool use = true;
+        if (a.act[f] != nullptr) {
+          // All slots of the target in one family share its block: test it once per run, and skip
+          // the run if no lane needs it.
+          const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
+          use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
+          if (__ballot(use) == 0ull) {
+            k = kr;
+            continue;
+          }
+        }
+        const float* part = a.partial[f];
+        const size_t pitch = (size_t)a.pitch[f];
+        if (a.prefix[f] == nullptr) {
+          // Split-major items: split j of cell k is item j * num_cells + k; lanes read consecutive
+          // cells of one split row.
+          const int uc = a.uniform_count[f];
+          const int total = (kr - k) * uc;
+          const bool live = my_k < kend && use;
+          int ent = k, jj = 0;  // (entry, split row) of value m
+          for (int m = 0; m < total; m += kReduceBatch) {
+            float v[kReduceBatch];
+#pragma unroll
+            for (int u = 0; u < kReduceBatch; ++u) {
+              v[u] = semi_zero<SEMI>();
+              if (m + u < total) {
+                const int sl = __builtin_amdgcn_readlane(ls, ent);
+                if (live) v[u] = part[(size_t)sl * pitch + (size_t)jj * a.num_cells + my_k];
+                if (++jj == uc) {
+                  jj = 0;
+                  ++ent;
+                }
+              }
+            }
             if (SEMI == kViterbi) {
-              acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
+#pragma unroll
+              for (int u = 0; u < kReduceBatch; u += 2) acc = fmaxf(acc, fmaxf(v[u], v[u + 1]));
             } else {
-              acc = semi_add<SEMI>(acc, v0);
-              acc = semi_add<SEMI>(acc, v1);
-              acc = semi_add<SEMI>(acc, v2);
-              acc = semi_add<SEMI>(acc, v3);
+#pragma unroll
+              for (int u = 0; u < kReduceBatch; ++u)
+                if (live && m + u < total) acc = semi_add<SEMI>(acc, v[u]);
             }
           }
-          for (; j < uc; ++j) acc = semi_add<SEMI>(acc, q[(size_t)j * a.num_cells]);
+          k = kr;
+          continue;
         }
-        continue;
-      }
-      int lo0, hi0, my_lo, my_hi;
-      {
-        const int32_t* pre = a.prefix[f];
-        const int base = pre[0];
-        lo0 = pre[k0] - base;
-        hi0 = pre[kend] - base;
-        my_lo = my_k < kend ? pre[my_k] - base : hi0;
-        my_hi = my_k < kend ? pre[my_k + 1] - base : hi0;
-      }
-      for (int pb = lo0; pb < hi0; pb += CAP) {
-        const int len = min(CAP, hi0 - pb);
+        int lo0, hi0, my_lo, my_hi;
+        {
+          const int32_t* pre = a.prefix[f];
+          const int base = pre[0];
+          lo0 = pre[k0] - base;
+          hi0 = pre[kend] - base;
+          my_lo = my_k < kend ? pre[my_k] - base : hi0;
+          my_hi = my_k < kend ? pre[my_k + 1] - base : hi0;
+        }
+        for (; k < kr; ++k) {
+          const float* p = part + (size_t)__builtin_amdgcn_readlane(ls, k) * pitch;
+          for (int pb = lo0; pb < hi0; pb += CAP) {
+            const int len = min(CAP, hi0 - pb);
 #pragma unroll
-        for (int u = 0; u < CAP / 64; ++u) {
-          const int i = lane + 64 * u;
-          if (i < len) buf[i] = p[pb + i];
+            for (int u = 0; u < CAP / 64; ++u) {
+              const int i = lane + 64 * u;
+              if (i < len) buf[i] = p[pb + i];
+            }
+            // LDS operations of one wave complete in order; the fences keep the compiler from
+            // moving the reads above the writes (or the next piece's writes above these reads).
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int jz = use ? min(my_hi, pb + len) : 0;
+            for (int j = max(my_lo, pb); j < jz; ++j) acc = semi_add<SEMI>(acc, buf[j - pb]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          }
         }
-        // LDS operations of one wave complete in order; the fences keep the compiler from moving
-        // the reads above the writes (or the next piece's writes above these reads).
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int jz = use ? min(my_hi, pb + len) : 0;
-        for (int j = max(my_lo, pb); j < jz; ++j) {
-          acc = semi_add<SEMI>(acc, buf[j - pb]);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
   }
