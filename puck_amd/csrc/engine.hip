@@ -394,12 +394,27 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   float acc[IPL];
 #pragma unroll
   for (int i = 0; i < IPL; ++i) acc[i] = zero;
+  // The flush's loop-invariant fields, moved into vector registers once per unit. Reloaded from
+  // the argument block inside the group loop (the compiler's choice under scalar-register
+  // pressure) they cost a scalar load whose wait also drains the LDS reads already in flight for
+  // the next group.
+  // Block-major (masked) streams keep the plain reads: their items are planned, never split pairs,
+  // and the moves measured slower there.
+  uint32_t out_lo = 0, out_hi = 0, pitch_v = 0, pair_v = 0;
+  if constexpr (!SKIP) {
+    const uint64_t base = reinterpret_cast<uint64_t>(a.partial);
+    asm volatile("v_mov_b32 %0, %1" : "=v"(out_lo) : "s"((uint32_t)base));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(out_hi) : "s"((uint32_t)(base >> 32)));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(pitch_v) : "s"((uint32_t)a.pitch));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(pair_v) : "s"((uint32_t)a.pair_splits));
+  }
   auto flush = [&](uint32_t tag, uint32_t slot) {
     // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
     const int bit = (int)((tag >> 24) & 127u);
-    float* out = a.partial + (size_t)slot * a.pitch;
-    if constexpr (IPL == 2) {
-      if (a.pair_splits) {
+    float* out = SKIP ? a.partial + (size_t)slot * a.pitch
+                      : reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot * pitch_v;
+    if constexpr (IPL == 2 && !SKIP) {
+      if (__builtin_amdgcn_readfirstlane(pair_v)) {
         float v[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -1740,7 +1755,8 @@ bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
 // the splits pair up without a lone last split or that lone split is a small share of the work.
 bool pairs_splits(const Layer& L, const DevFamily& F, int span) {
   const int splits = span - 1;
-  return L.ipl == 2 && F.kind == 0 && splits >= 2 && (splits % 2 == 0 || splits >= 15);
+  // Block-major streams (masked layers) never pair: their kernel variant writes one value per item.
+  return L.ipl == 2 && F.kind == 0 && !F.skip_layout && splits >= 2 && (splits % 2 == 0 || splits >= 15);
 }
 }  // namespace
 
