@@ -398,21 +398,19 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // the argument block inside the group loop (the compiler's choice under scalar-register
   // pressure) they cost a scalar load whose wait also drains the LDS reads already in flight for
   // the next group.
-  // Block-major (masked) streams keep the plain reads: their items are planned, never split pairs,
-  // and the moves measured slower there.
-  uint32_t out_lo = 0, out_hi = 0, pitch_v = 0, pair_v = 0;
-  if constexpr (!SKIP) {
+  // Block-major (masked) streams never pair splits (their items are planned).
+  uint32_t out_lo, out_hi, pitch_v, pair_v = 0;
+  {
     const uint64_t base = reinterpret_cast<uint64_t>(a.partial);
     asm volatile("v_mov_b32 %0, %1" : "=v"(out_lo) : "s"((uint32_t)base));
     asm volatile("v_mov_b32 %0, %1" : "=v"(out_hi) : "s"((uint32_t)(base >> 32)));
     asm volatile("v_mov_b32 %0, %1" : "=v"(pitch_v) : "s"((uint32_t)a.pitch));
-    asm volatile("v_mov_b32 %0, %1" : "=v"(pair_v) : "s"((uint32_t)a.pair_splits));
+    if constexpr (!SKIP) asm volatile("v_mov_b32 %0, %1" : "=v"(pair_v) : "s"((uint32_t)a.pair_splits));
   }
   auto flush = [&](uint32_t tag, uint32_t slot) {
     // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
     const int bit = (int)((tag >> 24) & 127u);
-    float* out = SKIP ? a.partial + (size_t)slot * a.pitch
-                      : reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot * pitch_v;
+    float* out = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot * pitch_v;
     if constexpr (IPL == 2 && !SKIP) {
       if (__builtin_amdgcn_readfirstlane(pair_v)) {
         float v[2];
