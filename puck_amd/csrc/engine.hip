@@ -118,6 +118,9 @@ __device__ inline float semi_add(float acc, float v) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// s_waitcnt immediate (gfx9 encoding) that waits for lgkmcnt 0 only: vmcnt 63, expcnt 7.
+constexpr int kWaitLgkm0 = 0xC07F;
+
 // Rule-family kernel. One workgroup = one tile of ITEMS items (target cell, split) x one chunk
 // (row range of operand A x row range of operand B). Operand rows are staged item-minor in LDS so
 // that a wave reads one row for IPL items per lane with one ds_read. Waves pull target symbols
@@ -129,9 +132,8 @@ __device__ inline float semi_add(float acc, float v) {
 template <int ITEMS>
 struct RulesLds {
   int32_t cell_a[ITEMS], cell_b[ITEMS];
-  uint32_t gate[4][ITEMS];  // 128 block bits per item
   float scale[ITEMS];
-  uint32_t uni[4];          // blocks active in any item of the tile
+  uint32_t uni[4];  // blocks active in any item of the tile (block-major streams)
 };
 
 // One work unit of a rule family: tile `tile` of ITEMS items x operand chunk `chunk` x piece
@@ -143,7 +145,6 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   constexpr int ITEMS = 64 * IPL;
   int32_t* s_cell_a = sh.cell_a;
   int32_t* s_cell_b = sh.cell_b;
-  uint32_t (*s_gate)[ITEMS] = sh.gate;
   float* s_scale = sh.scale;
   uint32_t* s_union = sh.uni;
   const float zero = semi_zero<SEMI>();
@@ -170,7 +171,9 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
 
   if (tid < ITEMS) {
     const int item = tile0 + tid;
-    int ca = 0, cb = 0;
+    // An invalid item (past the slice, or the missing partner of a lone last split pair) keeps
+    // cell -1: its operand rows are staged as the semiring zero, so its value stays zero.
+    int ca = -1, cb = -1;
     uint32_t gate[4] = {0u, 0u, 0u, 0u};
     float scale = 1.0f;
     int k = 0, j = 0;
@@ -213,33 +216,17 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       ca = cell_abs(a.cell_offsets, sent, n, a.a_half, ax0, ay0);
       cb = UNARY ? 0 : cell_abs(a.cell_offsets, sent, n, a.b_half, bx0, by0);
       const int ct = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
-      if (a.masks == nullptr) {
-        gate[0] = gate[1] = gate[2] = gate[3] = 0xffffffffu;
-      } else if (a.plan_items != nullptr) {
-        // Planned items have allowed children and a target with an allowed span; the plan kept
-        // the target cell's active blocks (a unary item's one block is the unary family).
-        if (UNARY) {
+      // Block-major streams walk the union of their items' active blocks: every block without
+      // masks; under masks (masked launches are always planned) the blocks the plan kept for the
+      // target cell, whose items have allowed children and an allowed target span.
+      if constexpr (SKIP) {
+        if (a.plan_items == nullptr) {
+          gate[0] = gate[1] = gate[2] = gate[3] = 0xffffffffu;
+        } else if (UNARY) {
           gate[0] = 1u;
         } else {
 #pragma unroll
           for (int w = 0; w < 4; ++w) gate[w] = a.plan_act[(size_t)w * a.num_cells + k];
-        }
-      } else if (UNARY) {
-        // UnaryUpdateManager.doUpdates gates on the top mask of the cell (CLParser.scala:747).
-        const int ctop = cell_abs(a.cell_offsets, sent, n, 1, b, e);
-        if (mask_any(a.masks + (size_t)ctop * a.mask_words, a.mask_words)) gate[0] = 1u;
-      } else {
-        // Children must be allowed spans (CLParser.scala:892-893) and the target cell's mask
-        // must intersect the block mask (CLParser.scala:845-855,867).
-        const bool ok_a = mask_any(a.masks + (size_t)ca * a.mask_words, a.mask_words);
-        const bool ok_b = mask_any(a.masks + (size_t)cb * a.mask_words, a.mask_words);
-        if (ok_a && ok_b) {
-          const uint32_t* tm = a.masks + (size_t)ct * a.mask_words;
-          for (int bb = 0; bb < a.num_blocks; ++bb) {
-            uint32_t hit = 0;
-            for (int w = 0; w < a.mask_words; ++w) hit |= tm[w] & a.block_masks[bb * a.mask_words + w];
-            if (hit) gate[bb >> 5] |= 1u << (bb & 31);
-          }
         }
       }
       if (SEMI == kReal) {
@@ -249,9 +236,9 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     }
     s_cell_a[tid] = ca;
     s_cell_b[tid] = cb;
-    for (int w = 0; w < 4; ++w) {
-      s_gate[w][tid] = gate[w];
-      if (SKIP && gate[w]) atomicOr(&s_union[w], gate[w]);
+    if constexpr (SKIP) {
+      for (int w = 0; w < 4; ++w)
+        if (gate[w]) atomicOr(&s_union[w], gate[w]);
     }
     s_scale[tid] = scale;
   }
@@ -274,7 +261,10 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
         const int idx = base + k * kBlockThreads;
         v[k] = zero;
         at[k] = row * a.row_stride + it;
-        if (idx < total && tile0 + it < num_items) v[k] = ch[(size_t)cells[it] * a.cell_size + lo + row];
+        if (idx < total) {
+          const int c = cells[it];
+          if (c >= 0) v[k] = ch[(size_t)c * a.cell_size + lo + row];
+        }
         row += step_row;
         it += step_it;
         if (row >= rows) {
@@ -361,16 +351,10 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     g1 = gi + GB;
   }
 
-  // Per-item gate words and scale, kept in registers for the slot flushes.
-  uint64_t gate_lo[IPL], gate_hi[IPL];  // block bits 0-63 and 64-127
+  // Per-item scale, kept in registers for the slot flushes.
   float scl[IPL];
 #pragma unroll
-  for (int i = 0; i < IPL; ++i) {
-    const int li = lane * IPL + i;
-    gate_lo[i] = (uint64_t)s_gate[0][li] | ((uint64_t)s_gate[1][li] << 32);
-    gate_hi[i] = (uint64_t)s_gate[2][li] | ((uint64_t)s_gate[3][li] << 32);
-    scl[i] = s_scale[li];
-  }
+  for (int i = 0; i < IPL; ++i) scl[i] = s_scale[lane * IPL + i];
   // Operand reads of one group: word u = A-row byte offset, word G + u = B-row byte offset.
   auto read_group = [&](const W16& off, float (*va)[IPL], float (*vb)[IPL]) {
 #pragma unroll
@@ -407,20 +391,28 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     asm volatile("v_mov_b32 %0, %1" : "=v"(pitch_v) : "s"((uint32_t)a.pitch));
     if constexpr (!SKIP) asm volatile("v_mov_b32 %0, %1" : "=v"(pair_v) : "s"((uint32_t)a.pair_splits));
   }
-  auto flush = [&](uint32_t tag, uint32_t slot) {
-    // Slot done: gate by the target cell's block bit, scale (Real) and write the partial.
-    const int bit = (int)((tag >> 24) & 127u);
-    float* out = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot * pitch_v;
+  // Plain streams flush their slots in order from p_slot on, so the partial row advances by one
+  // pitch per flush; block-major streams name each flushed slot.
+  float* row = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot_ctr * pitch_v;
+  auto flush = [&](uint32_t slot) {
+    // Slot done: scale (Real) and write the partial. Under masks, an item whose cell does not
+    // allow the slot's block also gets a value; the reduce never reads it, since it takes a
+    // (cell, target) partial only where the plan's active-block bits allow the target's block.
+    float v[IPL];
+#pragma unroll
+    for (int i = 0; i < IPL; ++i) {
+      v[i] = SEMI == kReal ? acc[i] * scl[i] : acc[i];
+      acc[i] = zero;
+    }
+    float* out;
+    if constexpr (SKIP) {
+      out = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot * pitch_v;
+    } else {
+      out = row;
+      row += pitch_v;
+    }
     if constexpr (IPL == 2 && !SKIP) {
       if (__builtin_amdgcn_readfirstlane(pair_v)) {
-        float v[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
-          v[i] = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
-          if (SEMI == kReal) v[i] *= scl[i];
-          acc[i] = zero;
-        }
         const int m = (tile0 >> 1) + lane;
         if (2 * m < num_items) out[m] = semi_pair<SEMI>(v[0], v[1]);
         return;
@@ -428,12 +420,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     }
 #pragma unroll
     for (int i = 0; i < IPL; ++i) {
-      const uint64_t word = (bit & 64) ? gate_hi[i] : gate_lo[i];
-      float v = ((word >> (bit & 63)) & 1u) ? acc[i] : zero;
-      if (SEMI == kReal) v *= scl[i];
       const int item = tile0 + lane * IPL + i;
-      if (item < num_items) out[item] = v;
-      acc[i] = zero;
+      if (item < num_items) out[item] = v[i];
     }
   };
   auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, float (*da)[IPL], float (*db)[IPL], int h) {
@@ -454,12 +442,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       }
     }
     if (tag & kSlotEnd) {
-      if constexpr (SKIP) {
-        flush(tag, slot);
-      } else {
-        flush(tag, slot_ctr);
-        ++slot_ctr;
-      }
+      flush(slot);
     }
   };
   // Software pipeline over groups: while group gi is computed from (da, db) with weights cur_q
@@ -471,6 +454,9 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   auto step = [&](const W8& cur_q, const W2& cur_s, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
                   uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q, W2& nxt_s,
                   float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
+    // Wait for the previous step's LDS reads and scalar loads before this step's reads issue; a
+    // wait the compiler places later (before the first use of da/db) would drain them as well.
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
     read_group(nxt_off, la, lb);
     nxt_t0 = nxt_off[H - 1];
     nxt_t1 = nxt_off[G - 1];
@@ -1780,6 +1766,7 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.cell_list = B.d_cells + B.span_start[span] + k0;
   a.num_cells = kc;
   int64_t nitems;
+  if (use_masks && plan_items == nullptr) throw std::logic_error("masked rule launches need a mask plan");
   a.plan_items = plan_items;
   a.plan_act = plan_act;
   a.slice_cell0 = 0;  // planned item codes are slice-relative
@@ -1817,8 +1804,6 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.sentinel_group = F.sentinel_group;
   a.rules = F.d_rules;
   a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
-  a.block_masks = F.unary ? nullptr : F.d_block_masks;
-  a.num_blocks = F.num_blocks;
   a.partial = partial;
   const int items_tile = 64 * L.ipl;
   a.pitch = (int32_t)(((a.pair_splits ? nitems / 2 : nitems) + 255) / 256 * 256);
@@ -1829,7 +1814,6 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.a_half = F.a_half;
   a.b_half = F.b_half;
   a.t_half = F.t_half;
-  a.mask_words = g.mask_words;
   a.row_stride = 64 * L.ipl + (L.ipl == 1 ? 1 : 2);
   a.num_chunks = F.num_chunks;
   // Few tiles: cut the rule streams into pieces so that the launch still fills the chip (about

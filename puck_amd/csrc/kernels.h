@@ -51,14 +51,12 @@ struct FamArgs {
   const int32_t* wave_blocks;
   const uint32_t* rules;  // rule groups, kGroupWords dwords each
   const int32_t* block_of_target;
-  const uint32_t* block_masks;
   float* partial;
   int32_t num_cells, num_items, uniform_count, slice_cell0;
   int32_t num_block_ranges, sentinel_group;
   int32_t pieces_log2;  // the launch cuts streams into 2^pieces_log2 pieces (grid z)
   int32_t span, kind, range_split, cell_size;
   int32_t a_half, b_half, t_half;
-  int32_t mask_words, num_blocks, unary_gate;
   int32_t pitch;
   int32_t row_stride;  // floats per LDS row
   int32_t num_chunks;  // operand chunks of the family (grid y of the plain launch)
