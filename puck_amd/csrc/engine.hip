@@ -244,37 +244,36 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   }
   __syncthreads();
 
-  // Stage operand rows: consecutive threads read consecutive symbols of one cell; eight loads are
-  // in flight per thread before the LDS stores.
-  // Element idx of the tile's rows is (item idx / rows, row idx % rows); a thread's elements are
-  // kBlockThreads apart, so it steps its (item, row) pair by a fixed quotient and remainder (one
-  // division per thread instead of one per element).
+  // Stage operand rows: wave w copies items w, w + 16, ... of the tile; its lanes take 64
+  // consecutive rows of an item's cell at a time, so each load is one contiguous run of the cell's
+  // rows from a wave-uniform base, and one row block's loads for all the wave's items are in flight
+  // together. An invalid item (cell -1) stages the semiring zero.
+  constexpr int kStageWaves = kBlockThreads / 64;
+  constexpr int IPW = ITEMS / kStageWaves;  // items per wave
+  static_assert(ITEMS % kStageWaves == 0, "items split evenly over the waves");
   auto stage = [&](const float* ch, const int32_t* cells, int lo, int rows, float* dst) {
-    const int total = rows * ITEMS;
-    const int step_it = kBlockThreads / rows, step_row = kBlockThreads - step_it * rows;
-    int it = tid / rows, row = tid - it * rows;
-    for (int base = tid; base < total; base += 8 * kBlockThreads) {
-      float v[8];
-      int at[8];
+    const int sw = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t sl = tid & 63;
+    const char* base[IPW];
+    bool ok[IPW];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int idx = base + k * kBlockThreads;
-        v[k] = zero;
-        at[k] = row * a.row_stride + it;
-        if (idx < total) {
-          const int c = cells[it];
-          if (c >= 0) v[k] = ch[(size_t)c * a.cell_size + lo + row];
-        }
-        row += step_row;
-        it += step_it;
-        if (row >= rows) {
-          row -= rows;
-          ++it;
-        }
+    for (int k = 0; k < IPW; ++k) {
+      const int c = __builtin_amdgcn_readfirstlane(cells[sw + kStageWaves * k]);
+      ok[k] = c >= 0;
+      base[k] = reinterpret_cast<const char*>(ch + (size_t)(ok[k] ? c : 0) * a.cell_size + lo);
+    }
+    for (int r0 = 0; r0 < rows; r0 += 64) {
+      const uint32_t row = (uint32_t)r0 + sl;
+      if (row < (uint32_t)rows) {
+        // A 32-bit byte offset from the uniform base: one load with a scalar base per item.
+        const uint32_t boff = row * 4u;
+        float v[IPW];
+#pragma unroll
+        for (int k = 0; k < IPW; ++k) v[k] = *reinterpret_cast<const float*>(base[k] + boff);
+        float* d = dst + row * a.row_stride + sw;
+#pragma unroll
+        for (int k = 0; k < IPW; ++k) d[kStageWaves * k] = ok[k] ? v[k] : zero;
       }
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (base + k * kBlockThreads < total) dst[at[k]] = v[k];
     }
   };
   stage(a.chA, s_cell_a, a_lo, rows_a, lds_a);
