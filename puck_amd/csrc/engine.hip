@@ -749,7 +749,25 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
             const float* p1 = base + (size_t)__builtin_amdgcn_readlane(ls, q + 1) * pitch;
             const float* p2 = base + (size_t)__builtin_amdgcn_readlane(ls, q + 2) * pitch;
             const float* p3 = base + (size_t)__builtin_amdgcn_readlane(ls, q + 3) * pitch;
-            for (int j = lo; j < hi; ++j) {
+            // Two items at a time: eight loads in flight, folded in (item, slot) order.
+            int j = lo;
+            for (; j + 2 <= hi; j += 2) {
+              const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
+              const float w0 = p0[j + 1], w1 = p1[j + 1], w2 = p2[j + 1], w3 = p3[j + 1];
+              if (SEMI == kViterbi) {
+                acc = fmaxf(fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3))), fmaxf(fmaxf(w0, w1), fmaxf(w2, w3)));
+              } else {
+                acc = semi_add<SEMI>(acc, v0);
+                acc = semi_add<SEMI>(acc, v1);
+                acc = semi_add<SEMI>(acc, v2);
+                acc = semi_add<SEMI>(acc, v3);
+                acc = semi_add<SEMI>(acc, w0);
+                acc = semi_add<SEMI>(acc, w1);
+                acc = semi_add<SEMI>(acc, w2);
+                acc = semi_add<SEMI>(acc, w3);
+              }
+            }
+            if (j < hi) {
               const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
               if (SEMI == kViterbi) {
                 acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
