@@ -262,17 +262,27 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       ok[k] = c >= 0;
       base[k] = reinterpret_cast<const char*>(ch + (size_t)(ok[k] ? c : 0) * a.cell_size + lo);
     }
-    for (int r0 = 0; r0 < rows; r0 += 64) {
-      const uint32_t row = (uint32_t)r0 + sl;
-      if (row < (uint32_t)rows) {
-        // A 32-bit byte offset from the uniform base: one load with a scalar base per item.
-        const uint32_t boff = row * 4u;
-        float v[IPW];
+    // Two row blocks per step: 2 x IPW loads in flight before the first LDS store.
+    for (int r0 = 0; r0 < rows; r0 += 128) {
+      const uint32_t row0 = (uint32_t)r0 + sl, row1 = row0 + 64u;
+      const bool in0 = row0 < (uint32_t)rows, in1 = row1 < (uint32_t)rows;
+      // 32-bit byte offsets from the uniform bases: one load with a scalar base per item. Lanes past
+      // the last row load that row (no branch between the loads) and store nothing.
+      const uint32_t b0 = min(row0, (uint32_t)rows - 1u) * 4u, b1 = min(row1, (uint32_t)rows - 1u) * 4u;
+      float v0[IPW], v1[IPW];
 #pragma unroll
-        for (int k = 0; k < IPW; ++k) v[k] = *reinterpret_cast<const float*>(base[k] + boff);
-        float* d = dst + row * a.row_stride + sw;
+      for (int k = 0; k < IPW; ++k) v0[k] = *reinterpret_cast<const float*>(base[k] + b0);
 #pragma unroll
-        for (int k = 0; k < IPW; ++k) d[kStageWaves * k] = ok[k] ? v[k] : zero;
+      for (int k = 0; k < IPW; ++k) v1[k] = *reinterpret_cast<const float*>(base[k] + b1);
+      if (in0) {
+        float* d = dst + row0 * a.row_stride + sw;
+#pragma unroll
+        for (int k = 0; k < IPW; ++k) d[kStageWaves * k] = ok[k] ? v0[k] : zero;
+      }
+      if (in1) {
+        float* d = dst + row1 * a.row_stride + sw;
+#pragma unroll
+        for (int k = 0; k < IPW; ++k) d[kStageWaves * k] = ok[k] ? v1[k] : zero;
       }
     }
   };
