@@ -123,11 +123,12 @@ constexpr int kWaitLgkm0 = 0xC07F;
 
 // Rule-family kernel. One workgroup = one tile of ITEMS items (target cell, split) x one chunk
 // (row range of operand A x row range of operand B). Operand rows are staged item-minor in LDS so
-// that a wave reads one row for IPL items per lane with one ds_read. Waves pull target symbols
-// (slots) of the chunk from an LDS counter and stream that slot's rules (uniform across the wave)
-// with buffer loads. Each slot writes one partial value per item; the reduce kernel combines
-// partials over splits, chunks and families. Rule semantics per SimpleGenRuleMultiply.java:151
-// (parent = semiring_mad(parent, left (x) right, q)) and :213-216 (Real: *= scale).
+// that a wave reads one row for IPL items per lane with one ds_read. Each wave streams its own
+// share of the chunk's target symbols (slots, assigned longest-first when the grammar is loaded):
+// the slots' rules, uniform across the wave, arrive in groups of eight by scalar loads. Each slot
+// writes one partial value per item; the reduce kernel combines partials over splits, chunks and
+// families. Rule semantics per SimpleGenRuleMultiply.java:151 (parent = semiring_mad(parent,
+// left (x) right, q)) and :213-216 (Real: *= scale).
 // Per-tile bookkeeping of the rule kernels in static LDS (the operand rows use dynamic LDS).
 template <int ITEMS>
 struct RulesLds {
