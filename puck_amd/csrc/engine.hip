@@ -104,9 +104,17 @@ __device__ inline float log_add(float x, float y) {
 // Semiring add of a lane's two paired splits in the rule kernel's flush.
 template <int SEMI>
 __device__ inline float semi_pair(float a, float b) {
-  if constexpr (SEMI == kViterbi) return fmaxf(a, b);
-  else if constexpr (SEMI == kLogSum) return log_add(a, b);
-  else return a + b;
+  // Viterbi: one v_max_f32 (fmaxf first quiets each operand, two more instructions; max-plus
+  // values are never NaN).
+  if constexpr (SEMI == kViterbi) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+  } else if constexpr (SEMI == kLogSum) {
+    return log_add(a, b);
+  } else {
+    return a + b;
+  }
 }
 
 // Semiring add of the reduce: max, +, or log-add.
