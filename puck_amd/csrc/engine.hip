@@ -246,8 +246,21 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     s_cell_a[tid] = ca;
     s_cell_b[tid] = cb;
     if constexpr (SKIP) {
-      for (int w = 0; w < 4; ++w)
-        if (gate[w]) atomicOr(&s_union[w], gate[w]);
+      // Union of the items' block bits: an OR across the wave (DPP row shifts, then row broadcasts
+      // that leave the total in lane 63), then one LDS atomic per wave and word instead of one per
+      // item on the same address. Every lane of the wave is active here (ITEMS is a multiple of 64).
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        int x = (int)gate[w];
+        x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+        x |= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+        x |= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+        x |= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+        x |= __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+        x |= __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+        const uint32_t all = (uint32_t)__builtin_amdgcn_readlane(x, 63);
+        if ((tid & 63) == 0 && all != 0u) atomicOr(&s_union[w], all);
+      }
     }
     s_scale[tid] = scale;
   }
