@@ -617,17 +617,26 @@ __global__ void k_plan(PlanArgs p) {
     }
   }
   if (!FILL) {
+    // Rule applications the reference semantics perform here: every rule of every active block
+    // for every item (CLParser.scala:899-901).
+    double rules = 0.0;
     if (count > 0) {
-      // Rule applications the reference semantics perform here: every rule of every active block
-      // for every item (CLParser.scala:899-901).
-      double rules = 0.0;
       if (f.unary) {
         rules = f.block_rules[0];
       } else {
         for (int bb = 0; bb < f.num_blocks; ++bb)
           if ((act[bb >> 5] >> (bb & 31)) & 1u) rules += f.block_rules[bb];
       }
-      atomicAdd(f.apps, rules * count);
+    }
+    double apps = rules * count;
+    // One atomic per full wave instead of one per cell on the same address: the lanes' counts are
+    // summed first (integers held exactly in doubles, so the order does not matter). A wave cut
+    // short by the slice's end keeps one atomic per cell.
+    if (__ballot(1) == ~0ull) {
+      for (int m = 32; m >= 1; m >>= 1) apps += __shfl_xor(apps, m);
+      if ((threadIdx.x & 63) == 0 && apps != 0.0) atomicAdd(f.apps, apps);
+    } else if (apps != 0.0) {
+      atomicAdd(f.apps, apps);
     }
     f.counts[k] = count;
     // A cell whose children all fail keeps no item: no block is active there.
