@@ -40,6 +40,26 @@ def test_viterbi_inside_chart_bit_exact(puck, wsj1, ref_wsj1, wsj1_sents):
         assert bad.size == 0, f"sentence {i}: cells {bad[:5]} differ"
 
 
+def test_lone_split_pairs_bit_exact(puck, wsj1, ref_wsj1):
+    """Spans with an odd number of splits (15 or more) fold split pairs in the rule kernel with a
+    lone last split, whose missing partner item must contribute the semiring zero. Inside charts
+    of sentences of length 16-21 bit-exact against the oracle, in the plain and persistent forms."""
+    from oracle.oracle import Oracle
+    words, offsets = wsj1.sample(seed=17, n=6, mode=0, min_len=16, max_len=21)
+    lens = np.diff(offsets)
+    assert (lens >= 16).all()
+    orc = Oracle(ref_wsj1)
+    refs = [orc.inside(words[offsets[i]:offsets[i + 1]])[0] for i in range(len(lens))]
+    for persistent in (False, True):
+        p = puck.CLParser([wsj1], mem_bytes=2 << 30, persistent=persistent)
+        ch = p.debug_charts((words, offsets), layer=0, with_outside=False)
+        for i, (c0, c1) in enumerate(sentence_slices(offsets)):
+            print("persistent", persistent, "sentence", i, "len", lens[i], flush=True)
+            got = ch["inside"][c0:c1]
+            bad = np.nonzero(~np.all(got.view(np.int32) == refs[i].view(np.int32), axis=1))[0]
+            assert bad.size == 0, f"persistent={persistent} sentence {i}: cells {bad[:5]} differ"
+
+
 def test_viterbi_trees_bit_exact(puck, wsj1, ref_wsj1, wsj1_sents):
     from oracle.oracle import Oracle
     words, offsets = wsj1_sents
