@@ -35,29 +35,7 @@
 #define REF_LOG logf
 #endif
 
-typedef struct {
-  int32_t n_b;
-  const int32_t *p, *l, *r;
-  const float *q, *w;
-} RefBin;
-
-typedef struct {
-  int32_t n_u;
-  const int32_t *p, *c;
-  const float *q, *w;
-} RefUn;
-
-typedef struct {
-  int32_t nN, nT, cell_size, num_syms, root, num_coarse, mask_words, semiring;
-  RefBin bin[4];   /* NN, NT, TN, TT */
-  RefUn un[2];     /* unary N child, unary T child */
-  const int32_t *proj_n, *proj_t;
-  /* Block gating: [which 0 inside, 1 out-left, 2 out-right][family]: block_of_coarse and masks. */
-  int32_t num_blocks[3][4];
-  const int32_t *block_of_coarse[3][4];
-  const uint32_t *block_masks[3][4];
-  int32_t final_layer;
-} RefGram;
+#include "cky_ref.h"
 
 static inline int tri_(int n) { return n * (n + 1) / 2; }
 static inline int cidx(int b, int e, int n) {
@@ -257,8 +235,11 @@ void SUFFIX(ref_outside)(const RefGram* g, int n, const REAL* in, REAL* out, con
   const int cs = g->cell_size;
   /* setRootScores */
   out[(size_t)(T + cidx(0, n, n)) * cs + g->root] = g->semiring == 1 ? (REAL)1 : (REAL)0;
-  /* NU at the root span */
-  apply_unary(g, 1, 0, out, cidx(0, n, n), out, T + cidx(0, n, n), out_scale, out_scale, masks, 0, n, n, 1);
+  /* NU at the root span. A one-word sentence gets TU there: the reference runs NU at the batch's
+   * longest span and TU in its loop at span 1 (CLParser.scala:328-341), which is what a one-word
+   * sentence receives in any batch that also holds a longer sentence. */
+  apply_unary(g, 1, n == 1 ? 1 : 0, out, cidx(0, n, n), out, T + cidx(0, n, n), out_scale, out_scale, masks, 0, n,
+              n, 1);
   for (int w = n - 1; w >= 1; --w) {
     for (int b = 0; b + w <= n; ++b) {
       const int e = b + w;

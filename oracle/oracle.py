@@ -229,6 +229,48 @@ def pipeline(layers, sentences, f64_final=False):
     return out
 
 
+class _Layer(ctypes.Structure):
+    _fields_ = [("g", ctypes.c_void_p), ("lex_off", _I32P), ("lex_tag", _I32P), ("lex_logp", _F64P)]
+
+
+def batch(layers, words, offsets, with_outside=False, threads=0):
+    """The coarse-to-fine fold of `pipeline` for a whole workload in C, parallel over sentences
+    (ref_parse_batch in cky_batch.c). Returns a dict: 'partitions' float32[n]; 'trees' (Viterbi
+    final layer) or 'mbr' (Real / LogSum) as per-sentence node arrays; 'masks' of the last coarse
+    layer (cells x mask_words, sentences concatenated) when there is one; 'threads' used."""
+    words = np.ascontiguousarray(words, np.int32)
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    lens = np.diff(offsets).astype(np.int64)
+    n = len(lens)
+    tree_off = np.concatenate([[0], np.cumsum(2 * lens - 1)]).astype(np.int64)
+    cell_off = np.concatenate([[0], np.cumsum(lens * (lens + 1))]).astype(np.int64)
+    descs = (_Layer * len(layers))()
+    for k, L in enumerate(layers):
+        r = L.ref
+        descs[k] = _Layer(ctypes.cast(ctypes.pointer(L.g), ctypes.c_void_p), _p(r.lex_off, _I32P),
+                          _p(r.lex_tag, _I32P), _p(r.lex_logp, _F64P))
+    final = layers[-1]
+    viterbi = final.ref.semiring == "viterbi"
+    off, sc, le, ri = final.dec
+    trees = np.zeros((max(int(tree_off[-1]), 1), 4), np.int32)
+    mbr = np.zeros((max(int(tree_off[-1]), 1), 5), np.int32)
+    parts = np.zeros(max(n, 1), np.float32)
+    mw = layers[0].ref.mask_words
+    masks = np.zeros((max(int(cell_off[-1]), 1), mw), np.uint32) if len(layers) > 1 else None
+    f = lib().ref_parse_batch
+    f.restype = ctypes.c_int
+    used = f(descs, len(layers), n, _p(words, _I32P), _p(offsets, _I32P), _p(off, _I32P), _p(sc, _F32P),
+             _p(le, _I32P), _p(ri, _I32P), None if masks is None else _p(masks, _U32P), _p(cell_off, ctypes.POINTER(ctypes.c_int64)),
+             _p(trees, _I32P) if viterbi else None, None if viterbi else _p(mbr, _I32P),
+             _p(tree_off, ctypes.POINTER(ctypes.c_int64)), _p(parts, _F32P), int(with_outside), int(threads))
+    nodes = trees if viterbi else mbr
+    out = {"partitions": parts[:n], "threads": int(used),
+           "trees" if viterbi else "mbr": [nodes[tree_off[i]:tree_off[i + 1]].copy() for i in range(n)]}
+    if masks is not None:
+        out["masks"] = masks[:int(cell_off[-1])]
+    return out
+
+
 # -- brute force over tree shapes (tiny inputs) ------------------------------------------------
 def brute_force_root(ref: RefGrammar, words, semiring="viterbi"):
     """Best (or summed) score of ROOT over all binary bracketings of the sentence, computed per tree
