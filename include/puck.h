@@ -134,9 +134,11 @@ enum puck_parser_flags {
   PUCK_FLAG_PERSISTENT = 2,        /* rule families of a span share one persistent work-queue launch */
   PUCK_FLAG_GRAPHS = 4,            /* parse calls capture each batch shape's device work into a HIP
                                       graph on its second sighting and replay it afterwards */
-  PUCK_FLAG_NO_PLAN_AHEAD = 8      /* masked passes plan each group on the parser's one stream instead
-                                      of ahead on a second one: for several parsers working side by
-                                      side in one process, whose streams may share hardware queues */
+  PUCK_FLAG_NO_PLAN_AHEAD = 8,     /* accepted and ignored: every parser issues all of its device work,
+                                      mask plans included, on its one HIP stream */
+  PUCK_FLAG_OUTSIDE_FINAL = 16     /* puck_parse_viterbi also runs the final layer's outside pass before
+                                      the decode, as CLParser.insideOutside does
+                                      (CLParser.scala:171-182); the trees do not change */
 };
 /* new CLParser(data, maxAllocSize) (CLParser.scala:46-48): mem_bytes bounds the device workspace,
  * as the reference's --mem option bounds WorkSpace.allocate (WorkSpace.scala:91-120). */
@@ -193,6 +195,7 @@ typedef struct {
   int32_t graphs;         /* per-shape HIP graphs */
   int32_t piece_target;   /* see puck_launch_config_t */
   int32_t items_per_lane; /* 1 or 2 */
+  int32_t outside_final;  /* Viterbi final layer: outside pass before the decode (insideOutside) */
 } puck_batch_config_t;
 typedef struct {
   double device_ms; /* first kernel of the first bucket to the last kernel of the last bucket */

@@ -28,8 +28,8 @@ BatchParser::BatchParser(std::vector<std::shared_ptr<Grammar>> layers, int32_t d
     Parser& p = *parsers_.back();
     p.timing = false;
     p.set_persistent(cfg_.persistent);
-    p.set_plan_ahead(false);  // decided per parse: see parse()
     p.set_launch_shape(cfg_.piece_target, 1);
+    p.set_outside_final(cfg_.outside_final);
     p.use_graphs = cfg_.graphs;
   }
   hip_ok(hipSetDevice(device_), "hipSetDevice");
@@ -84,10 +84,6 @@ void BatchParser::parse(const int32_t* words, const int32_t* offsets, int32_t n,
       plan[w].push_back(b);
     }
   }
-  // A worker plans masked passes ahead on a second stream only when it runs alone; workers running
-  // side by side use one stream each (the plan and engine streams of several busy parsers have
-  // stalled each other for good).
-  for (int32_t w = 0; w < (int32_t)parsers_.size(); ++w) parsers_[w]->set_plan_ahead(S == 1 && w == 0);
   for (auto& p : parsers_) p->ref_event = ref_;
   hip_ok(hipEventRecord(ref_, nullptr), "hipEventRecord");
   hip_ok(hipEventSynchronize(ref_), "hipEventSynchronize");

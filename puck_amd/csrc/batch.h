@@ -27,6 +27,7 @@ struct BatchConfig {
   bool graphs = true;             // capture each bucket shape's device work into a HIP graph
   int32_t piece_target = 512;
   int32_t items_per_lane = 2;
+  bool outside_final = false;     // Viterbi final layer: outside pass before the decode (insideOutside)
 };
 
 struct BatchStats {

@@ -75,7 +75,7 @@ int64_t nodes_needed(const int32_t* offsets, int32_t n) {
 extern "C" {
 
 const char* puck_last_error(void) { return g_last_error.c_str(); }
-int32_t puck_abi_version(void) { return 1; }
+int32_t puck_abi_version(void) { return 2; }
 
 int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out) {
   return guarded([&]() -> int32_t {
@@ -298,7 +298,7 @@ int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers
       h->p.reset(new puck::Parser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, ipl));
       h->p->set_persistent((flags & PUCK_FLAG_PERSISTENT) != 0);
       h->p->use_graphs = (flags & PUCK_FLAG_GRAPHS) != 0;
-      h->p->set_plan_ahead((flags & PUCK_FLAG_NO_PLAN_AHEAD) == 0);
+      h->p->set_outside_final((flags & PUCK_FLAG_OUTSIDE_FINAL) != 0);
     } catch (...) {
       delete h;
       throw;
@@ -540,6 +540,7 @@ int32_t puck_batch_create(const puck_grammar_t* const* layers, int32_t n_layers,
     c.graphs = cfg->graphs != 0;
     c.piece_target = cfg->piece_target;
     c.items_per_lane = cfg->items_per_lane;
+    c.outside_final = cfg->outside_final != 0;
     auto* h = new puck_batch;
     try {
       h->b.reset(new puck::BatchParser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, c));

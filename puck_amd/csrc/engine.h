@@ -107,6 +107,7 @@ class Layer {
   int32_t ipl;  // items per lane of the rule kernels (1 or 2)
   DevFamily fam[kNumFamilies];
   DevReduce red[kNumReduce];
+  PlanFamStatic* d_plan_static = nullptr;  // mask plan data of every family (kNumFamilies)
   // Decoder tables.
   int32_t* d_dec_offsets = nullptr;
   float* d_dec_scores = nullptr;
@@ -196,17 +197,14 @@ class Parser {
     persistent = on;
     drop_graphs();
   }
-  // Masked passes plan every span up front on a second stream (created on first use), so that
-  // plan kernels of later spans run beside earlier spans' rule launches. The two streams wait on
-  // each other (plans read the masks; rule launches read the plans). This is for a parser that has
-  // the device to itself: three busy parsers doing so side by side in one process (six such
-  // streams over the process's four hardware queues) stalled for good. Off, each masked group
-  // plans on the engine stream right before its launch.
-  void set_plan_ahead(bool on) {
-    if (plan_ahead_ == on) return;
-    plan_ahead_ = on;
-    drop_graphs();  // captured graphs hold the other stream layout
+  // parse() with a Viterbi final layer also runs that layer's outside pass before the decode, as
+  // CLParser.insideOutside does (CLParser.scala:171-182); the trees do not depend on it.
+  void set_outside_final(bool on) {
+    if (outside_final == on) return;
+    outside_final = on;
+    drop_graphs();
   }
+  bool outside_final = false;
   // Drops every captured graph (their launches no longer match the configuration).
   void drop_graphs();
   int32_t piece_target_ = 512;
@@ -233,8 +231,6 @@ class Parser {
   int32_t device_;
   int64_t mem_bytes_;
   hipStream_t stream_ = nullptr;
-  bool plan_ahead_ = true;
-  hipStream_t plan_stream_ = nullptr;  // mask plans of a masked pass, ahead of its rule launches
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // apps_slot >= 0: the launch's rule applications add the plan counts in
   // apps_[apps_slot, apps_slot + apps_count).
@@ -255,7 +251,7 @@ class Parser {
   // Workspace (grown on demand).
   struct Buf { void* p = nullptr; size_t bytes = 0; };
   Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
-      pos_scale_, aux_, mplan_, apps_, queue_;
+      pos_scale_, aux_, mplan_, apps_, queue_, jobs_;
   int32_t queue_next_ = 0;
   int32_t num_cus_ = 256;
   void ensure(Buf& b, size_t bytes);
@@ -290,22 +286,33 @@ class Parser {
                     const std::vector<const int32_t*>& plan_counts,
                     const std::vector<const int32_t*>& plan_items,
                     const std::vector<const uint32_t*>& plan_acts, const std::vector<int32_t>& apps_slots);
-  // Mask plan of one slice of a group's cells at one span.
+  // Host view of the mask plan of one slice of a group's cells at one span: per family of the
+  // group, device pointers into the plan buffer (see PlanJob).
+  struct PlanPtrs {
+    int32_t* pre = nullptr;
+    int32_t* items = nullptr;
+    uint32_t* act = nullptr;
+    int32_t* posof = nullptr;
+    int32_t* perm = nullptr;
+  };
   struct PlannedSlice {
     int32_t k0 = 0, kc = 0;
-    std::vector<PlanFam> pfs;
+    std::vector<PlanPtrs> pfs;
     std::vector<int32_t> apps_slot;
-    int32_t* d_perm = nullptr;
-    hipEvent_t ready = nullptr;  // recorded on plan_stream_ after the slice's plan kernels
+    int32_t* d_perm = nullptr;  // visiting order of the reduce: the group's largest family's order
   };
   std::vector<std::pair<int32_t, int32_t>> group_slices(Batch& b, int32_t li, int32_t rid, int32_t span);
   int64_t family_items(Batch& b, int32_t li, int32_t fid, int32_t span, int32_t k0, int32_t kc);
-  size_t plan_ints(Batch& b, int32_t li, int32_t rid, int32_t span, int32_t k0, int32_t kc);
-  PlannedSlice plan_slice(Batch& b, int32_t li, int32_t rid, int32_t span, int32_t k0, int32_t kc,
-                          int32_t* mp, hipStream_t st);
-  void plan_pass(Batch& b, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
-                 std::vector<std::vector<PlannedSlice>>* out);
-  void run_groups(Batch& b, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups, bool use_masks);
+  // Passes of a call in execution order: (layer, outside pass). `what` as in run_batch.
+  std::vector<std::pair<int32_t, bool>> pass_schedule(int32_t what, int32_t chart_layer) const;
+  static std::vector<std::pair<int32_t, int32_t>> inside_groups(const Batch& b);
+  static std::vector<std::pair<int32_t, int32_t>> outside_groups(const Batch& b);
+  // Lays out the mask plans of every masked pass of the call (before any device work, so that the
+  // plan job table is uploaded with the batch and graphs replay it unchanged).
+  void prepare_plans(Batch& b, const std::vector<std::pair<int32_t, bool>>& schedule);
+  // The four plan launches over every job of layer li's masked passes, on the parser's stream.
+  void launch_plans(Batch& b, int32_t li);
+  void run_groups(Batch& b, int32_t li, bool outside, bool use_masks);
   void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks,
                  const std::vector<PlannedSlice>* planned);
   void timed_begin();

@@ -568,21 +568,47 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules_q(QueueArgs q) {
   }
 }
 
-// Mask plan of one span: the items each family must compute under the pruning masks, as the
+// Mask plans (see PlanJob): the items each family must compute under the pruning masks, as the
 // reference's update managers enumerate them (CLParser.scala:838-925): a binary item needs a
 // target cell whose mask meets some block of the family (:845-855) and allowed child spans
-// (:892-893); a unary item needs a non-empty top mask (:747). Counts, then an exclusive scan per
-// family, then the item lists ((cell << 8) | split, cell-major).
+// (:892-893); a unary item needs a non-empty top mask (:747). Counts, a stable sort of each job's
+// cells by key, an exclusive scan per job in that order, then the item lists ((cell << 8) | split).
+// The count and fill launches flatten every job's cells into one grid of kPlanThreads-thread
+// blocks (job j owns blocks [block0_j, block0_{j+1})); sort and scan take one workgroup per job.
+struct PlanView {
+  int32_t *keys, *perm, *posof, *counts, *pre, *items;
+  uint32_t* act;
+};
+
+__device__ inline PlanView plan_view(int32_t* mplan, const PlanJob& J) {
+  const int64_t kc = J.num_cells;
+  int32_t* b = mplan + J.mp;
+  return PlanView{b, b + kc, b + 2 * kc, b + 3 * kc, b + 4 * kc, b + 9 * kc + 1,
+                  reinterpret_cast<uint32_t*>(b + 5 * kc + 1)};
+}
+
+// Job owning flattened block `blk`: the last job whose block0 <= blk.
+__device__ inline int plan_job_of(const PlanJob* jobs, int num_jobs, int blk) {
+  int lo = 0, hi = num_jobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (jobs[mid].block0 <= blk) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 template <bool FILL>
-__global__ void k_plan(PlanArgs p) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const PlanFam& f = p.fam[blockIdx.y];
-  if (k >= p.num_cells) return;
-  const int packed = p.cell_list[k];
-  const int sent = packed >> 8, b = packed & 255, e = b + p.span;
+__global__ __launch_bounds__(kPlanThreads) void k_plan(PlanBatchArgs p) {
+  const PlanJob& J = p.jobs[plan_job_of(p.jobs, p.num_jobs, blockIdx.x)];
+  const PlanFamStatic& f = p.fams[J.fid];
+  const PlanView v = plan_view(p.mplan, J);
+  const int k = (blockIdx.x - J.block0) * kPlanThreads + threadIdx.x;
+  if (k >= J.num_cells) return;
+  const int packed = p.cells[J.cells_start + k];
+  const int sent = packed >> 8, b = packed & 255, e = b + J.span;
   const int n = p.lengths[sent];
   int count = 0;
-  int32_t* out = FILL ? f.items + f.pre[f.posof[k]] : nullptr;
+  int32_t* out = FILL ? v.items + v.pre[v.posof[k]] : nullptr;
   uint32_t act[4] = {0u, 0u, 0u, 0u};
   if (f.unary) {
     if (mask_any(p.masks + (size_t)cell_abs(p.cell_offsets, sent, n, 1, b, e) * p.mask_words, p.mask_words)) {
@@ -632,16 +658,17 @@ __global__ void k_plan(PlanArgs p) {
     // One atomic per full wave instead of one per cell on the same address: the lanes' counts are
     // summed first (integers held exactly in doubles, so the order does not matter). A wave cut
     // short by the slice's end keeps one atomic per cell.
+    double* counter = p.apps + J.apps_slot;
     if (__ballot(1) == ~0ull) {
       for (int m = 32; m >= 1; m >>= 1) apps += __shfl_xor(apps, m);
-      if ((threadIdx.x & 63) == 0 && apps != 0.0) atomicAdd(f.apps, apps);
+      if ((threadIdx.x & 63) == 0 && apps != 0.0) atomicAdd(counter, apps);
     } else if (apps != 0.0) {
-      atomicAdd(f.apps, apps);
+      atomicAdd(counter, apps);
     }
-    f.counts[k] = count;
+    v.counts[k] = count;
     // A cell whose children all fail keeps no item: no block is active there.
 #pragma unroll
-    for (int w = 0; w < 4; ++w) f.act[(size_t)w * p.num_cells + k] = count ? act[w] : 0u;
+    for (int w = 0; w < 4; ++w) v.act[(size_t)w * J.num_cells + k] = count ? act[w] : 0u;
     // Sort key: the family's largest blocks active here, the largest as the most significant bit.
     int key = 0;
     if (count > 0) {
@@ -651,33 +678,34 @@ __global__ void k_plan(PlanArgs p) {
         if (bb >= 0 && ((act[bb >> 5] >> (bb & 31)) & 1u)) key |= 1 << (kPlanKeyBlocks - 1 - i);
       }
     }
-    f.keys[k] = key;
-    // A family without key blocks keeps the slice order (no sort pass runs for it).
+    v.keys[k] = key;
+    // A family without key blocks keeps the slice order (its sort returns at once).
     if (f.key_blocks[0] < 0) {
-      f.perm[k] = k;
-      f.posof[k] = k;
+      v.perm[k] = k;
+      v.posof[k] = k;
     }
   }
 }
 
-// Stable sort of the slice's cells by the family's key (one workgroup per family): two counting-
-// sort passes over 4-bit digits, low digit first. Pass one writes (key << 24 | cell) in low-digit
-// order into posof, pass two scatters cells by the high digit into perm, then posof is rebuilt
-// as perm's inverse.
-__global__ __launch_bounds__(512) void k_plan_sort(PlanArgs p) {
+// Stable sort of a job's cells by key (one workgroup per job): two counting-sort passes over 4-bit
+// digits, low digit first. Pass one writes (key << 24 | cell) in low-digit order into posof, pass
+// two scatters cells by the high digit into perm, then posof is rebuilt as perm's inverse.
+__global__ __launch_bounds__(512) void k_plan_sort(PlanBatchArgs p) {
   constexpr int T = 512, D = 16;
   static_assert(kPlanKeyBlocks == 8, "two 4-bit digits");
   __shared__ int32_t s_a[D * T];
   __shared__ int32_t s_sum[T];
-  const PlanFam& f = p.fam[blockIdx.y];
-  if (f.key_blocks[0] < 0) return;  // order already written by k_plan
+  const PlanJob& J = p.jobs[blockIdx.x];
+  if (p.fams[J.fid].key_blocks[0] < 0) return;  // order already written by the count pass
+  const PlanView v = plan_view(p.mplan, J);
+  const int nc = J.num_cells;
   const int t = threadIdx.x;
-  const int per = (p.num_cells + T - 1) / T;
-  const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
-  int32_t* tmp = f.posof;
+  const int per = (nc + T - 1) / T;
+  const int lo = min(t * per, nc), hi = min(lo + per, nc);
+  int32_t* tmp = v.posof;
   for (int pass = 0; pass < 2; ++pass) {
     auto digit = [&](int i) -> int {
-      return pass == 0 ? (f.keys[i] & (D - 1)) : ((tmp[i] >> 28) & (D - 1));
+      return pass == 0 ? (v.keys[i] & (D - 1)) : ((tmp[i] >> 28) & (D - 1));
     };
     for (int b = 0; b < D; ++b) s_a[b * T + t] = 0;
     for (int i = lo; i < hi; ++i) ++s_a[digit(i) * T + t];
@@ -689,54 +717,54 @@ __global__ __launch_bounds__(512) void k_plan_sort(PlanArgs p) {
     s_sum[t] = local;
     __syncthreads();
     for (int d = 1; d < T; d <<= 1) {
-      const int32_t v = t >= d ? s_sum[t - d] : 0;
+      const int32_t x = t >= d ? s_sum[t - d] : 0;
       __syncthreads();
-      s_sum[t] += v;
+      s_sum[t] += x;
       __syncthreads();
     }
     int32_t run = s_sum[t] - local;
     for (int i = 0; i < D; ++i) {
-      const int32_t v = s_a[t * D + i];
+      const int32_t x = s_a[t * D + i];
       s_a[t * D + i] = run;
-      run += v;
+      run += x;
     }
     __syncthreads();
     if (pass == 0) {
-      for (int k = lo; k < hi; ++k) tmp[s_a[digit(k) * T + t]++] = (f.keys[k] << 24) | k;
+      for (int k = lo; k < hi; ++k) tmp[s_a[digit(k) * T + t]++] = (v.keys[k] << 24) | k;
     } else {
-      for (int i = lo; i < hi; ++i) f.perm[s_a[digit(i) * T + t]++] = tmp[i] & 0xffffff;
+      for (int i = lo; i < hi; ++i) v.perm[s_a[digit(i) * T + t]++] = tmp[i] & 0xffffff;
     }
     __syncthreads();
   }
-  for (int i = lo; i < hi; ++i) f.posof[f.perm[i]] = i;
+  for (int i = lo; i < hi; ++i) v.posof[v.perm[i]] = i;
 }
 
-// Exclusive scan of the item counts in the sorted cell order into pre[0, num_cells]; one block per
-// family.
-__global__ __launch_bounds__(1024) void k_plan_scan(PlanArgs p) {
+// Exclusive scan of a job's item counts in its sorted cell order into pre[0, num_cells]; one
+// workgroup per job.
+__global__ __launch_bounds__(1024) void k_plan_scan(PlanBatchArgs p) {
   __shared__ int32_t s_sum[1024];
-  const int32_t* c = p.fam[blockIdx.y].counts;
-  const int32_t* perm = p.fam[blockIdx.y].perm;
-  int32_t* pre = p.fam[blockIdx.y].pre;
+  const PlanJob& J = p.jobs[blockIdx.x];
+  const PlanView v = plan_view(p.mplan, J);
+  const int nc = J.num_cells;
   const int t = threadIdx.x;
-  const int per = (p.num_cells + 1023) / 1024;
-  const int lo = min(t * per, p.num_cells), hi = min(lo + per, p.num_cells);
+  const int per = (nc + 1023) / 1024;
+  const int lo = min(t * per, nc), hi = min(lo + per, nc);
   int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += c[perm[i]];
+  for (int i = lo; i < hi; ++i) sum += v.counts[v.perm[i]];
   s_sum[t] = sum;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {
-    const int32_t v = t >= d ? s_sum[t - d] : 0;
+    const int32_t x = t >= d ? s_sum[t - d] : 0;
     __syncthreads();
-    s_sum[t] += v;
+    s_sum[t] += x;
     __syncthreads();
   }
   int32_t run = s_sum[t] - sum;  // exclusive prefix of this thread's range
   for (int i = lo; i < hi; ++i) {
-    pre[i] = run;
-    run += c[perm[i]];
+    v.pre[i] = run;
+    run += v.counts[v.perm[i]];
   }
-  if (t == 1023) pre[p.num_cells] = s_sum[1023];
+  if (t == 1023) v.pre[nc] = s_sum[1023];
 }
 
 // Combine partials: target[cell][t] = (+) over (family, slot) of t over the cell's items.
@@ -1311,6 +1339,21 @@ Layer::Layer(Grammar gr, bool final_l, bool masked_l, int32_t items_per_lane)
   d_lex_offsets = upload(g.lex_offsets);
   d_lex_tags = upload(g.lex_tags);
   d_lex_logp = upload(g.lex_logp);
+  std::vector<PlanFamStatic> pst(kNumFamilies);
+  for (int f = 0; f < kNumFamilies; ++f) {
+    const DevFamily& F = fam[f];
+    PlanFamStatic& ps = pst[f];
+    ps.kind = F.kind;
+    ps.unary = F.unary;
+    ps.a_half = F.a_half;
+    ps.b_half = F.b_half;
+    ps.t_half = F.t_half;
+    ps.num_blocks = F.unary ? 0 : F.num_blocks;
+    ps.block_masks = F.unary ? nullptr : F.d_block_masks;
+    ps.block_rules = F.d_block_rules;
+    for (int i = 0; i < kPlanKeyBlocks; ++i) ps.key_blocks[i] = i < (int)F.h_key_blocks.size() ? F.h_key_blocks[i] : -1;
+  }
+  d_plan_static = upload(pst);
   std::vector<int32_t> pn = g.proj_n, pt = g.proj_t;
   pn.resize(g.num_syms, 0);
   pt.resize(g.num_syms, 0);
@@ -1658,6 +1701,19 @@ struct Parser::Batch {
   std::vector<int32_t> h_prefix;
   std::vector<double> partition_scales;
   size_t apps_cap = 0;   // plan counters (doubles) of the batch; queue heads use 3x as many ints
+  // Mask plans of the call's masked passes, in schedule order (prepare_plans); run_groups takes
+  // them in turn from next_pass. Jobs of each masked layer are contiguous in the job table.
+  struct PassPlan {
+    int32_t li = 0;
+    bool outside = false;
+    std::vector<std::vector<PlannedSlice>> groups;  // per group of the pass (inside_groups order)
+  };
+  std::vector<PassPlan> pass_plans;
+  size_t next_pass = 0;
+  struct LayerJobs { int32_t li; int32_t first_job, num_jobs, total_blocks; bool any_key; };
+  std::vector<LayerJobs> layer_jobs;
+  std::vector<PlanJob> h_jobs;
+  PlanJob* d_jobs = nullptr;
   int32_t mask_in = 0;   // which half of masks_ holds the current input masks
   bool has_masks = false;
   bool has_scales = false;
@@ -1711,13 +1767,12 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
 
 Parser::~Parser() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
-                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_})
+                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_})
     if (b->p) (void)hipFree(b->p);
   drop_graphs();
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
-  if (plan_stream_) (void)hipStreamDestroy(plan_stream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -2090,113 +2145,161 @@ int64_t Parser::family_items(Batch& B, int32_t li, int32_t fid, int32_t span, in
   return B.h_prefix[po + k0 + kc] - B.h_prefix[po + k0];
 }
 
-// Ints of mask-plan memory one slice needs: per family keys, perm, posof, counts, prefix,
-// active-block bits and the item list.
-size_t Parser::plan_ints(Batch& B, int32_t li, int32_t rid, int32_t span, int32_t k0, int32_t kc) {
-  size_t total = 0;
-  for (int fid : layers_[li]->red[rid].families)
-    total += 9 * (size_t)kc + 1 + (size_t)std::max<int64_t>(family_items(B, li, fid, span, k0, kc), 1);
-  return total;
+// Passes of a call in execution order (run_batch with `what`, parse_device for what = 0).
+std::vector<std::pair<int32_t, bool>> Parser::pass_schedule(int32_t what, int32_t chart_layer) const {
+  std::vector<std::pair<int32_t, bool>> s;
+  const int32_t nl = (int32_t)layers_.size();
+  const bool viterbi_final = layers_.back()->g.semiring == kViterbi;
+  if (what == 0) {
+    for (int32_t li = 0; li < nl; ++li) {
+      s.push_back({li, false});
+      if (li + 1 < nl || !viterbi_final || outside_final) s.push_back({li, true});
+    }
+    return s;
+  }
+  const int32_t last_layer = (what == 3) ? nl - 2 : nl - 1;
+  for (int32_t li = 0; li <= last_layer; ++li) {
+    s.push_back({li, false});
+    if (li != nl - 1) {
+      s.push_back({li, true});
+      if (what == 4 && li == chart_layer) break;
+      continue;
+    }
+    if (what == 1) break;
+    if (!viterbi_final || what == 2 || what == 4) s.push_back({li, true});
+  }
+  return s;
 }
 
-// Mask plan of one slice (device): per family counts, active-block bits and the rule applications,
-// the family's cell order, and the prefix and item list in it. Launches on stream `st`, in `mp`.
-Parser::PlannedSlice Parser::plan_slice(Batch& B, int32_t li, int32_t rid, int32_t span, int32_t k0,
-                                        int32_t kc, int32_t* mp, hipStream_t st) {
-  Layer& L = *layers_[li];
-  const DevReduce& R = L.red[rid];
-  const Grammar& g = L.g;
-  const size_t nfam = R.families.size();
-  if (kc >= (1 << 23)) throw std::runtime_error("mask plan slice exceeds 2^23 cells");
-  PlannedSlice ps;
-  ps.k0 = k0;
-  ps.kc = kc;
-  ps.pfs.resize(nfam);
-  ps.apps_slot.assign(nfam, -1);
-  int key_fam = 0;
-  for (size_t fi = 1; fi < nfam; ++fi)
-    if (L.fam[R.families[fi]].num_rules > L.fam[R.families[key_fam]].num_rules) key_fam = (int)fi;
-  size_t o = 0;
-  PlanArgs pa{};
-  for (size_t fi = 0; fi < nfam; ++fi) {
-    const DevFamily& F = L.fam[R.families[fi]];
-    PlanFam& pf = pa.fam[fi];
-    pf.kind = F.kind;
-    pf.unary = F.unary;
-    pf.a_half = F.a_half;
-    pf.b_half = F.b_half;
-    pf.t_half = F.t_half;
-    pf.num_blocks = F.unary ? 0 : F.num_blocks;
-    pf.block_masks = F.unary ? nullptr : F.d_block_masks;
-    pf.keys = mp + o;
-    pf.perm = mp + o + kc;
-    pf.posof = mp + o + 2 * (size_t)kc;
-    o += 3 * (size_t)kc;
-    pf.counts = mp + o;
-    o += (size_t)kc;
-    pf.pre = mp + o;
-    o += (size_t)kc + 1;
-    pf.act = (uint32_t*)(mp + o);
-    o += (size_t)4 * kc;
-    pf.items = mp + o;
-    o += (size_t)std::max<int64_t>(family_items(B, li, R.families[fi], span, k0, kc), 1);
-    for (int i = 0; i < kPlanKeyBlocks; ++i)
-      pf.key_blocks[i] = i < (int)F.h_key_blocks.size() ? F.h_key_blocks[i] : -1;
-    pf.block_rules = F.d_block_rules;
-    if ((size_t)(apps_next_ + 1) * 8 > apps_.bytes) throw std::runtime_error("plan counter buffer full");
-    ps.apps_slot[fi] = apps_next_;
-    pf.apps = (double*)apps_.p + apps_next_++;
-    ps.pfs[fi] = pf;
+// The (reduce group, span) list of a pass in schedule order (CLParser.scala:256-369): inside, TU at
+// span 1 then per span the binary families and NU; outside, per span from the root down the binary
+// families then NU (at span 1 TU with the terminal binary families).
+std::vector<std::pair<int32_t, int32_t>> Parser::inside_groups(const Batch& B) {
+  std::vector<std::pair<int32_t, int32_t>> groups{{kRInUT, 1}};
+  for (int w = 2; w <= B.max_len; ++w) {
+    groups.push_back({w == 2 ? kRInBin2 : kRInBin, w});
+    groups.push_back({kRInUN, w});
   }
-  ps.d_perm = pa.fam[key_fam].perm;
-  pa.masks = (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * g.mask_words;
-  pa.cell_offsets = B.d_cell_offsets;
-  pa.lengths = B.d_lengths;
-  pa.cell_list = B.d_cells + B.span_start[span] + k0;
-  pa.num_cells = kc;
-  pa.span = span;
-  pa.mask_words = g.mask_words;
-  const dim3 pg((kc + 255) / 256, (unsigned)nfam);
-  hipLaunchKernelGGL((k_plan<false>), pg, dim3(256), 0, st, pa);
-  bool any_key = false;
-  for (size_t fi = 0; fi < nfam; ++fi) any_key |= pa.fam[fi].key_blocks[0] >= 0;
-  if (any_key) hipLaunchKernelGGL(k_plan_sort, dim3(1, (unsigned)nfam), dim3(512), 0, st, pa);
-  hipLaunchKernelGGL(k_plan_scan, dim3(1, (unsigned)nfam), dim3(1024), 0, st, pa);
-  hipLaunchKernelGGL((k_plan<true>), pg, dim3(256), 0, st, pa);
-  HIP_CHECK(hipGetLastError());
-  return ps;
+  return groups;
 }
 
-// Plans every (span, group) of a masked pass up front on the plan stream, so that the small plan
-// kernels of later spans run beside the rule and reduce launches of earlier ones; each group's
-// rule launch waits for its own plan only.
-void Parser::plan_pass(Batch& B, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
-                       std::vector<std::vector<PlannedSlice>>* out) {
-  size_t total = 0;
-  std::vector<std::vector<std::pair<int32_t, int32_t>>> slices(groups.size());
-  for (size_t k = 0; k < groups.size(); ++k) {
-    const auto [rid, span] = groups[k];
-    if (B.span_count[span] == 0) continue;
-    slices[k] = group_slices(B, li, rid, span);
-    for (auto [k0, kc] : slices[k]) total += plan_ints(B, li, rid, span, k0, kc);
+std::vector<std::pair<int32_t, int32_t>> Parser::outside_groups(const Batch& B) {
+  std::vector<std::pair<int32_t, int32_t>> groups;
+  for (int w = B.max_len; w >= 1; --w) {
+    groups.push_back({kROutBin, w});
+    groups.push_back({w >= 2 ? kROutUN : kROutW1, w});
   }
-  ensure(mplan_, std::max<size_t>(total, 1) * 4);
-  if (plan_stream_ == nullptr) HIP_CHECK(hipStreamCreateWithFlags(&plan_stream_, hipStreamNonBlocking));
-  hipEvent_t fork = next_event();
-  HIP_CHECK(hipEventRecord(fork, stream_));
-  HIP_CHECK(hipStreamWaitEvent(plan_stream_, fork, 0));
-  int32_t* mp = (int32_t*)mplan_.p;
-  out->assign(groups.size(), {});
-  for (size_t k = 0; k < groups.size(); ++k) {
-    const auto [rid, span] = groups[k];
-    for (auto [k0, kc] : slices[k]) {
-      PlannedSlice ps = plan_slice(B, li, rid, span, k0, kc, mp, plan_stream_);
-      mp += plan_ints(B, li, rid, span, k0, kc);
-      ps.ready = next_event();
-      HIP_CHECK(hipEventRecord(ps.ready, plan_stream_));
-      (*out)[k].push_back(std::move(ps));
+  return groups;
+}
+
+// Lays out one plan job per (masked pass, group, slice, family): its arrays in the plan buffer,
+// its blocks in its layer's flattened count and fill grids, and its plan counter. Every layer but
+// the first runs under masks; a layer's jobs are planned together at the start of its inside pass.
+void Parser::prepare_plans(Batch& B, const std::vector<std::pair<int32_t, bool>>& schedule) {
+  B.pass_plans.clear();
+  B.layer_jobs.clear();
+  B.h_jobs.clear();
+  B.next_pass = 0;
+  struct Pending { size_t pass, group, slice, fi; int64_t mp; };
+  std::vector<Pending> where;
+  int64_t mp = 0;
+  for (auto [li, outside] : schedule) {
+    if (li == 0) continue;  // the first layer has no masks
+    Layer& L = *layers_[li];
+    if (B.layer_jobs.empty() || B.layer_jobs.back().li != li)
+      B.layer_jobs.push_back({li, (int32_t)B.h_jobs.size(), 0, 0, false});
+    auto& lj = B.layer_jobs.back();
+    Batch::PassPlan pp;
+    pp.li = li;
+    pp.outside = outside;
+    const auto groups = outside ? outside_groups(B) : inside_groups(B);
+    pp.groups.resize(groups.size());
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const auto [rid, span] = groups[gi];
+      if (B.span_count[span] == 0) continue;
+      const DevReduce& R = L.red[rid];
+      for (auto [k0, kc] : group_slices(B, li, rid, span)) {
+        if (kc >= (1 << 23)) throw std::runtime_error("mask plan slice exceeds 2^23 cells");
+        PlannedSlice ps;
+        ps.k0 = k0;
+        ps.kc = kc;
+        ps.pfs.resize(R.families.size());
+        ps.apps_slot.resize(R.families.size());
+        for (size_t fi = 0; fi < R.families.size(); ++fi) {
+          const int fid = R.families[fi];
+          const DevFamily& F = L.fam[fid];
+          PlanJob j{};
+          j.fid = fid;
+          j.num_cells = kc;
+          j.span = span;
+          j.cells_start = B.span_start[span] + k0;
+          j.block0 = lj.total_blocks;
+          j.apps_slot = (int32_t)B.h_jobs.size();
+          j.mp = mp;
+          lj.total_blocks += (kc + kPlanThreads - 1) / kPlanThreads;
+          lj.any_key |= !F.h_key_blocks.empty();
+          ps.apps_slot[fi] = j.apps_slot;
+          where.push_back({B.pass_plans.size(), gi, pp.groups[gi].size(), fi, mp});
+          mp += 9 * (int64_t)kc + 1 + std::max<int64_t>(family_items(B, li, fid, span, k0, kc), 1);
+          B.h_jobs.push_back(j);
+          ++lj.num_jobs;
+        }
+        pp.groups[gi].push_back(std::move(ps));
+      }
+    }
+    B.pass_plans.push_back(std::move(pp));
+  }
+  if (B.h_jobs.empty()) return;
+  ensure(mplan_, (size_t)mp * 4);
+  ensure(jobs_, B.h_jobs.size() * sizeof(PlanJob));
+  int32_t* base = (int32_t*)mplan_.p;
+  for (const Pending& w : where) {
+    PlannedSlice& ps = B.pass_plans[w.pass].groups[w.group][w.slice];
+    const int64_t kc = ps.kc;
+    int32_t* b = base + w.mp;
+    PlanPtrs& pf = ps.pfs[w.fi];
+    pf.perm = b + kc;
+    pf.posof = b + 2 * kc;
+    pf.pre = b + 4 * kc;
+    pf.act = (uint32_t*)(b + 5 * kc + 1);
+    pf.items = b + 9 * kc + 1;
+  }
+  // The reduce visits a slice's cells in the order of its group's largest family.
+  for (Batch::PassPlan& pp : B.pass_plans) {
+    const auto groups = pp.outside ? outside_groups(B) : inside_groups(B);
+    const Layer& L = *layers_[pp.li];
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const DevReduce& R = L.red[groups[gi].first];
+      int key_fam = 0;
+      for (size_t fi = 1; fi < R.families.size(); ++fi)
+        if (L.fam[R.families[fi]].num_rules > L.fam[R.families[key_fam]].num_rules) key_fam = (int)fi;
+      for (PlannedSlice& ps : pp.groups[gi]) ps.d_perm = ps.pfs[key_fam].perm;
     }
   }
+}
+
+void Parser::launch_plans(Batch& B, int32_t li) {
+  const Batch::LayerJobs* lj = nullptr;
+  for (const auto& x : B.layer_jobs)
+    if (x.li == li) lj = &x;
+  if (lj == nullptr || lj->num_jobs == 0) return;
+  const Layer& L = *layers_[li];
+  PlanBatchArgs pa{};
+  pa.jobs = B.d_jobs + lj->first_job;
+  pa.fams = L.d_plan_static;
+  pa.mplan = (int32_t*)mplan_.p;
+  pa.apps = (double*)apps_.p;
+  pa.masks = (const uint32_t*)masks_.p + (size_t)B.mask_in * B.num_cells * L.g.mask_words;
+  pa.cell_offsets = B.d_cell_offsets;
+  pa.lengths = B.d_lengths;
+  pa.cells = B.d_cells;
+  pa.num_jobs = lj->num_jobs;
+  pa.mask_words = L.g.mask_words;
+  hipLaunchKernelGGL((k_plan<false>), dim3(lj->total_blocks), dim3(kPlanThreads), 0, stream_, pa);
+  if (lj->any_key) hipLaunchKernelGGL(k_plan_sort, dim3(lj->num_jobs), dim3(512), 0, stream_, pa);
+  hipLaunchKernelGGL(k_plan_scan, dim3(lj->num_jobs), dim3(1024), 0, stream_, pa);
+  hipLaunchKernelGGL((k_plan<true>), dim3(lj->total_blocks), dim3(kPlanThreads), 0, stream_, pa);
+  HIP_CHECK(hipGetLastError());
 }
 
 void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use_masks,
@@ -2209,24 +2312,16 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
   const size_t nfam = R.families.size();
   auto slice_items = [&](int fid, int k0, int kc) -> int64_t { return family_items(B, li, fid, span, k0, kc); };
   const auto slices = group_slices(B, li, rid, span);
-  if (planned != nullptr && planned->size() != slices.size()) throw std::logic_error("plan slices differ");
   for (size_t si = 0; si < slices.size(); ++si) {
     const int32_t k0 = slices[si].first, kc = slices[si].second;
     const int32_t* cells = B.d_cells + B.span_start[span] + k0;
-    PlannedSlice own;
     const PlannedSlice* ps = nullptr;
     if (use_masks) {
-      if (planned != nullptr) {
-        ps = &(*planned)[si];
-        HIP_CHECK(hipStreamWaitEvent(stream_, ps->ready, 0));
-      } else {
-        ensure(mplan_, plan_ints(B, li, rid, span, k0, kc) * 4);
-        own = plan_slice(B, li, rid, span, k0, kc, (int32_t*)mplan_.p, stream_);
-        ps = &own;
-      }
+      if (planned == nullptr || planned->size() != slices.size()) throw std::logic_error("masked group without its plan");
+      ps = &(*planned)[si];
     }
-    const std::vector<PlanFam> empty;
-    const std::vector<PlanFam>& pfs = ps ? ps->pfs : empty;
+    const std::vector<PlanPtrs> empty;
+    const std::vector<PlanPtrs>& pfs = ps ? ps->pfs : empty;
     const std::vector<int32_t> no_slots(nfam, -1);
     const std::vector<int32_t>& apps_slot = ps ? ps->apps_slot : no_slots;
     int32_t* d_perm = ps ? ps->d_perm : nullptr;
@@ -2336,14 +2431,9 @@ void Parser::inside_pass(Batch& B, int32_t li, bool use_masks) {
                        (double*)pos_scale_.p, g.cell_size);
   });
   HIP_CHECK(hipGetLastError());
-  // The pass's groups in schedule order (CLParser.scala:256-314): TU at span 1, then per span the
-  // binary families and NU.
-  std::vector<std::pair<int32_t, int32_t>> groups{{kRInUT, 1}};
-  for (int w = 2; w <= B.max_len; ++w) {
-    groups.push_back({w == 2 ? kRInBin2 : kRInBin, w});
-    groups.push_back({kRInUN, w});
-  }
-  run_groups(B, li, groups, use_masks);
+  // A masked layer plans every masked pass of its own up front (the plans read only the masks).
+  if (use_masks) launch_plans(B, li);
+  run_groups(B, li, false, use_masks);
 }
 
 void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
@@ -2352,29 +2442,21 @@ void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
   const float zero = zero_of(g.semiring);
   const size_t nfl = (size_t)B.num_cells * g.cell_size;
   hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, (float*)outside_.p, nfl, zero);
-  // CLParser.scala:316-369: per span from the root down, the binary families then NU (TU at 1).
-  std::vector<std::pair<int32_t, int32_t>> groups;
-  for (int w = B.max_len; w >= 1; --w) {
-    groups.push_back({kROutBin, w});
-    groups.push_back({w >= 2 ? kROutUN : kROutW1, w});
-  }
-  run_groups(B, li, groups, use_masks);
+  run_groups(B, li, true, use_masks);
 }
 
-// Runs a pass's (group, span) list in order. Under masks the plans of every group are made first
-// on the plan stream (plan_pass), and the plan stream is joined back at the end.
-void Parser::run_groups(Batch& B, int32_t li, const std::vector<std::pair<int32_t, int32_t>>& groups,
-                        bool use_masks) {
-  if (!use_masks || !plan_ahead_) {
-    for (auto [rid, w] : groups) run_group(B, li, rid, w, use_masks, nullptr);
-    return;
+// Runs a pass's (group, span) list in order, each masked group with its slices' plans.
+void Parser::run_groups(Batch& B, int32_t li, bool outside, bool use_masks) {
+  const auto groups = outside ? outside_groups(B) : inside_groups(B);
+  const Batch::PassPlan* pp = nullptr;
+  if (use_masks) {
+    if (B.next_pass >= B.pass_plans.size() || B.pass_plans[B.next_pass].li != li ||
+        B.pass_plans[B.next_pass].outside != outside)
+      throw std::logic_error("masked pass out of the planned schedule");
+    pp = &B.pass_plans[B.next_pass++];
   }
-  std::vector<std::vector<PlannedSlice>> plans;
-  plan_pass(B, li, groups, &plans);
-  for (size_t k = 0; k < groups.size(); ++k) run_group(B, li, groups[k].first, groups[k].second, true, &plans[k]);
-  hipEvent_t join = next_event();
-  HIP_CHECK(hipEventRecord(join, plan_stream_));
-  HIP_CHECK(hipStreamWaitEvent(stream_, join, 0));
+  for (size_t k = 0; k < groups.size(); ++k)
+    run_group(B, li, groups[k].first, groups[k].second, use_masks, pp ? &pp->groups[k] : nullptr);
 }
 
 // what: 0 = parse, 1 = partitions, 2 = insideOutside, 3 = masks of the last coarse layer,
@@ -2469,10 +2551,18 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   ensure(pos_scale_, (size_t)B.total_len * 8);
   const size_t partial_bytes = std::max<size_t>((size_t)(mem_bytes_ * 0.3), (size_t)64 << 20);
   ensure(partial_, partial_bytes);
-  // Plan counters: one double per planned (layer, span, family group member) of the batch.
-  // (Four reduce groups per span and layer at most: inside binary and unary, outside binary and
-  // unary; room for eight partial-buffer slices each.)
-  B.apps_cap = (size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies * 8;
+  // Mask plans of every masked pass of the call, laid out now so that the job table travels with
+  // the batch's other uploads (and a captured graph replays it unchanged).
+  prepare_plans(B, pass_schedule(what, chart_layer));
+  if (!B.h_jobs.empty()) {
+    HIP_CHECK(hipMemcpyAsync(jobs_.p, B.h_jobs.data(), B.h_jobs.size() * sizeof(PlanJob), hipMemcpyHostToDevice,
+                             stream_));
+    B.d_jobs = (PlanJob*)jobs_.p;
+  }
+  // Plan counters: one double per plan job. Queue heads: one per persistent launch, at most three
+  // layouts per (layer, pass, group, slice); sized with room for eight slices per group.
+  B.apps_cap = std::max<size_t>((size_t)layers_.size() * 4 * (B.max_len + 2) * kMaxGroupFamilies * 8,
+                                B.h_jobs.size());
   ensure(apps_, B.apps_cap * 8);
   // Heads of the persistent rule queues: one per queue launch of the batch (at most three layouts
   // per reduce group and slice).
@@ -2610,8 +2700,9 @@ void Parser::reset_counters(Batch& B) {
   hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)apps_.p, B.apps_cap * 2, 0.0f);
   if (persistent)
     hipLaunchKernelGGL(k_fill, dim3(16), dim3(256), 0, stream_, (float*)queue_.p, B.apps_cap * 3, 0.0f);
-  apps_next_ = 0;
+  apps_next_ = (int32_t)B.h_jobs.size();  // every plan job owns one counter
   queue_next_ = 0;
+  B.next_pass = 0;
 }
 
 // Device work of CLParser.parse on one batch: every layer's passes, the masks between layers and
@@ -2636,6 +2727,12 @@ void Parser::parse_device(Batch& B) {
       continue;
     }
     if (g.semiring == kViterbi) {
+      if (outside_final) {
+        // CLParser.insideOutside (CLParser.scala:171-182): the outside pass before the decode.
+        timed_begin();
+        outside_pass(B, li, B.has_masks);
+        timed_end(&stats.total_ms);
+      }
       timed_begin();
       launch_viterbi(B, li);
       timed_end(&stats.other_ms);

@@ -99,33 +99,49 @@ struct RedArgs {
   float one;
 };
 
-// Mask plan of one slice of a span for the families of a reduce group. Each family puts the cells
-// in its own order: a stable radix sort by which of the family's kPlanKeyBlocks largest blocks
-// the cell allows (the largest block is the most significant bit), so that a tile's items share
-// their active blocks and the rule kernel's walk over the union of its items' blocks wastes
-// little.
+// Mask plans. A plan job is one family of one partial-buffer slice of one (reduce group, span) of
+// a masked pass: the items the family must compute under the pruning masks, in the family's own
+// cell order. Each family orders its cells by a stable radix sort on which of its kPlanKeyBlocks
+// largest blocks the cell allows (the largest block is the most significant bit), so that a tile's
+// items share their active blocks and the rule kernel's walk over the union of its items' blocks
+// wastes little. Every job of a layer's masked passes is planned by the same four launches (count,
+// sort, scan, fill) on the parser's stream, the jobs spread over the grids.
 constexpr int kPlanKeyBlocks = 8;
-struct PlanFam {
+constexpr int kPlanThreads = 256;  // threads per block of the count and fill launches
+// Per-family data of a layer that every job of the family shares.
+struct PlanFamStatic {
   int32_t kind, unary, a_half, b_half, t_half, num_blocks;
   const uint32_t* block_masks;
-  int32_t* counts;  // num_cells: items per cell
-  int32_t* pre;     // num_cells + 1: item prefix in the family's cell order
-  int32_t* items;   // planned items in the family's cell order, (cell << 8) | split
-  uint32_t* act;    // active-block bits per cell: 4 rows of num_cells words
-  int32_t* keys;    // num_cells: sort key of each cell
-  int32_t* perm;    // num_cells: sorted position -> cell
-  int32_t* posof;   // num_cells: cell -> sorted position
   const double* block_rules;  // rules per block (binary) or the rule count (unary, one entry)
-  double* apps;     // accumulates the rule applications the plan implies
   int32_t key_blocks[kPlanKeyBlocks];  // the family's largest blocks, largest first (-1: none)
 };
-struct PlanArgs {
-  PlanFam fam[kMaxGroupFamilies];
+// One job. Its arrays lie back to back at `mp` ints into the plan buffer (kc = num_cells):
+//   keys[kc]   sort key of each cell
+//   perm[kc]   sorted position -> cell
+//   posof[kc]  cell -> sorted position
+//   counts[kc] items per cell
+//   pre[kc+1]  item prefix in the family's cell order
+//   act[4 kc]  active-block bits per cell, 4 rows of kc words
+//   items[..]  planned items in the family's cell order, (cell << 8) | split
+struct PlanJob {
+  int32_t fid;          // family id within the layer
+  int32_t num_cells;
+  int32_t span;
+  int32_t cells_start;  // first cell of the slice in the batch's cell list
+  int32_t block0;       // first block of this job in the flattened count and fill grids
+  int32_t apps_slot;    // plan counter: the rule applications the plan implies
+  int64_t mp;
+};
+struct PlanBatchArgs {
+  const PlanJob* jobs;
+  const PlanFamStatic* fams;
+  int32_t* mplan;
+  double* apps;
   const uint32_t* masks;
   const int32_t* cell_offsets;
   const int32_t* lengths;
-  const int32_t* cell_list;  // cells of the slice
-  int32_t num_cells, span, mask_words;
+  const int32_t* cells;
+  int32_t num_jobs, mask_words;
 };
 
 }  // namespace puck

@@ -25,7 +25,6 @@ Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
     parsers_.emplace_back(new Parser(layers, device, per_slot, items_per_lane));
     parsers_.back()->timing = false;
     parsers_.back()->set_persistent(persistent);
-    parsers_.back()->set_plan_ahead(false);  // slots share the device: one stream each
   }
   for (int32_t i = 0; i < max_inflight; ++i) threads_.emplace_back(&Server::worker, this, i);
 }

@@ -66,8 +66,22 @@ def test_masked_real_partitions_and_mbr(puck, wsj0, wsj1_real, c2f_sents, oracle
                                    rtol=1e-3, atol=1e-4)
 
 
-def test_planning_on_one_stream_same_trees(puck, wsj0, wsj1, c2f_sents):
-    """Plans made ahead on a second stream and plans made on the engine stream give the same trees."""
-    ahead = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True).parse(c2f_sents)
-    inline = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True, plan_ahead=False).parse(c2f_sents)
-    assert all(np.array_equal(a, b) for a, b in zip(ahead, inline))
+def test_parsers_side_by_side_same_trees(puck, wsj0, wsj1, c2f_sents):
+    """Two parsers working at once from two threads (one HIP stream each, mask plans included)
+    return the trees each returns alone."""
+    import threading
+    want = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True).parse(c2f_sents)
+    ps = [puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True) for _ in range(2)]
+    got = [None, None]
+
+    def run(k):
+        for _ in range(3):
+            got[k] = ps[k].parse(c2f_sents)
+
+    ts = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for k in range(2):
+        assert all(np.array_equal(a, b) for a, b in zip(got[k], want)), f"parser {k}"
