@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -175,6 +176,12 @@ class Parser {
                     bool outside, Charts* out);
 
   PassStats stats;
+  // Device bytes the parser's workspace holds now (every buffer it has grown).
+  int64_t workspace_bytes() const;
+  // Device bytes one call on sentences of these lengths makes the workspace hold (an upper bound
+  // that counts every masked item as planned): charts, scales, masks, mask plans, the partial
+  // buffer and the batch tables. Serving admits requests by it.
+  int64_t bytes_needed(const std::vector<int32_t>& lengths) const;
   // When set, each batch's device interval is also measured from this event (recorded on any
   // stream of the same device before the call): stats.span_lo_ms / span_hi_ms.
   hipEvent_t ref_event = nullptr;
@@ -205,6 +212,12 @@ class Parser {
     drop_graphs();
   }
   bool outside_final = false;
+  // Size of the rule kernels' partial buffer (default: 30% of mem_bytes, at least 64 MiB); a
+  // smaller buffer cuts a span's cells into more slices, never changing results.
+  void set_partial_bytes(int64_t bytes) {
+    partial_bytes_ = std::max<int64_t>(bytes, (int64_t)64 << 20);
+    drop_graphs();
+  }
   // Drops every captured graph (their launches no longer match the configuration).
   void drop_graphs();
   int32_t piece_target_ = 512;
@@ -230,6 +243,7 @@ class Parser {
   std::vector<std::unique_ptr<Layer>> layers_;
   int32_t device_;
   int64_t mem_bytes_;
+  int64_t partial_bytes_ = 0;  // 0: 30% of mem_bytes_
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // apps_slot >= 0: the launch's rule applications add the plan counts in

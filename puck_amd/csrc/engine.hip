@@ -1808,6 +1808,39 @@ void Parser::ensure(Buf& b, size_t bytes) {
   b.bytes = bytes;
 }
 
+int64_t Parser::workspace_bytes() const {
+  int64_t total = 0;
+  for (const Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_, &trees_,
+                       &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_})
+    total += (int64_t)b->bytes;
+  return total;
+}
+
+int64_t Parser::bytes_needed(const std::vector<int32_t>& lengths) const {
+  int32_t max_cell = 0;
+  for (auto& l : layers_) max_cell = std::max(max_cell, l->g.cell_size);
+  const int64_t mw = layers_[0]->g.mask_words;
+  const bool masked = layers_.size() > 1;
+  int64_t cells = 0, words = 0, plan_ints = 0, tables = 0;
+  for (int32_t n64 : lengths) {
+    const int64_t n = n64, c = n * (n + 1), tri_n = c / 2, c3 = (n + 1) * n * (n - 1) / 6;
+    cells += c;
+    words += n;
+    tables += 4 * n + 2 + c / 2 + 8 * (tri_n + 1);
+    if (masked) {
+      // Per masked layer: six inside and ten outside plan jobs per span-cell at most, and every
+      // item of their families (NN over all splits, the others one or a range per cell).
+      const int64_t in_items = c3 + n * (n - 1) + (n - 1) + 2 * tri_n;
+      const int64_t out_items = 2 * c3 + 2 * n * (n - 1) + 2 * n + c;
+      plan_ints += (int64_t)(layers_.size() - 1) * (9 * 16 * tri_n + in_items + out_items);
+    }
+  }
+  const int64_t partial = partial_bytes_ > 0 ? partial_bytes_
+                                            : std::max<int64_t>((int64_t)(mem_bytes_ * 0.3), (int64_t)64 << 20);
+  return cells * (2 * (int64_t)max_cell * 4 + 8 + 8 * mw + (int64_t)sizeof(MbrCell)) + words * 8 + 4 * plan_ints +
+         4 * tables + partial + ((int64_t)16 << 20);
+}
+
 // WorkSpace.getBatches (WorkSpace.scala:20-41): close a batch when its chart cells exceed the
 // chart budget.
 std::vector<std::pair<int32_t, int32_t>> Parser::make_batches(const int32_t* offsets, int32_t n) {
@@ -2549,7 +2582,8 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   ensure(out_scale_, (size_t)B.num_cells * 4);
   ensure(masks_, (size_t)2 * B.num_cells * mask_words * 4);
   ensure(pos_scale_, (size_t)B.total_len * 8);
-  const size_t partial_bytes = std::max<size_t>((size_t)(mem_bytes_ * 0.3), (size_t)64 << 20);
+  const size_t partial_bytes = partial_bytes_ > 0 ? (size_t)partial_bytes_
+                                                 : std::max<size_t>((size_t)(mem_bytes_ * 0.3), (size_t)64 << 20);
   ensure(partial_, partial_bytes);
   // Mask plans of every masked pass of the call, laid out now so that the job table travels with
   // the batch's other uploads (and a captured graph replays it unchanged).

@@ -1,6 +1,7 @@
 // grammar.grz container: zip (stored entries) of binary grammar tables. See grz.h.
 #include "grz.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -14,7 +15,7 @@ namespace puck {
 namespace {
 
 constexpr uint32_t kMagic = 0x5a524750;  // "PGRZ"
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
 
 uint32_t crc32(const std::string& s) {
   static uint32_t table[256];
@@ -145,8 +146,7 @@ std::string grammar_payload(const Grammar& g) {
   return w.b;
 }
 
-// parser-i/structure: CoarseParent partitions (inside, outside-left, outside-right per family) and
-// the Viterbi decoder tables.
+// parser-i/structure: CoarseParent partitions (inside, outside-left, outside-right per family).
 std::string structure_payload(const Grammar& g) {
   Out w;
   w.pod(kMagic);
@@ -154,11 +154,62 @@ std::string structure_payload(const Grammar& g) {
   for (int f = 0; f < 4; ++f) put_blocks(w, g.inside_blocks[f]);
   for (int f = 0; f < 4; ++f) put_blocks(w, g.outside_l_blocks[f]);
   for (int f = 0; f < 4; ++f) put_blocks(w, g.outside_r_blocks[f]);
-  w.vec(g.dec_offsets);
-  w.vec(g.dec_scores);
-  w.vec(g.dec_left);
-  w.vec(g.dec_right);
   return w.b;
+}
+
+// Raw little-endian array payload (the Viterbi table entries and the rule-block lists).
+template <class T>
+std::string raw(const std::vector<T>& v) {
+  return std::string(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(T));
+}
+
+template <class T>
+std::vector<T> unraw(const std::string& b, const std::string& entry) {
+  if (b.size() % sizeof(T)) throw std::runtime_error("bad grz payload: " + entry + " size");
+  std::vector<T> v(b.size() / sizeof(T));
+  if (!v.empty()) std::memcpy(v.data(), b.data(), b.size());
+  return v;
+}
+
+// What a kernel entry holds here: the device code lives in libpuck.so, so `<name>.name` is the
+// kernel that replaces the reference's and `<name>.program` names the code object.
+void kernel_entry(std::vector<std::pair<std::string, std::string>>& e, const std::string& name,
+                  const std::string& kernel) {
+  e.push_back({name + ".program", "gfx950 code object in libpuck.so"});
+  e.push_back({name + ".name", kernel});
+}
+
+// A rule-kernel bundle (CLBinaryRuleUpdater.write / CLUnaryRuleUpdater.write,
+// CLBinaryRuleUpdater.scala:60-67, CLUnaryRuleUpdater.scala:34-39): numKernels = the CoarseParent
+// blocks of the family, then per block its coarse-symbol mask (bits), the indices of its rules in
+// the family's rule table (rules), the workgroup shape and the kernel; binary bundles also carry
+// the work-list kernel (enqueuer: the mask plan), extra and directWrite.
+void bundle_entries(std::vector<std::pair<std::string, std::string>>& e, const std::string& name,
+                    const BlockPartition* bp, const std::vector<int32_t>& rule_coarse, int32_t mask_words) {
+  const int32_t nb = bp ? std::max<int32_t>(bp->num_blocks, 1) : 1;
+  e.push_back({name + "/numKernels", std::to_string(nb)});
+  for (int32_t j = 0; j < nb; ++j) {
+    const std::string d = name + "/" + std::to_string(j);
+    std::vector<int32_t> rules;
+    for (size_t r = 0; r < rule_coarse.size(); ++r)
+      if (!bp || bp->num_blocks == 0 || bp->block_of_coarse[rule_coarse[r]] == j) rules.push_back((int32_t)r);
+    std::vector<uint32_t> bits(mask_words, 0u);
+    if (bp && bp->num_blocks > 0)
+      for (int32_t w = 0; w < mask_words; ++w) bits[w] = bp->masks[(size_t)j * mask_words + w];
+    else
+      for (int32_t w = 0; w < mask_words; ++w) bits[w] = ~0u;
+    kernel_entry(e, d + "/0", "k_rules_q");
+    e.push_back({d + "/entries", d + "/0"});
+    e.push_back({d + "/wgSize", "1024"});
+    e.push_back({d + "/bits", raw(bits)});
+    e.push_back({d + "/rules", raw(rules)});
+  }
+  if (bp) {
+    kernel_entry(e, name + "/enqueuer/enqueue", "k_plan");
+    e.push_back({name + "/enqueuer/ints", raw(std::vector<int32_t>{mask_words})});
+    e.push_back({name + "/extra", ""});
+    e.push_back({name + "/directWrite", "false"});
+  }
 }
 
 void check_header(In& r, const std::string& entry) {
@@ -205,10 +256,6 @@ void read_structure(const std::string& b, Grammar& g) {
   for (int f = 0; f < 4; ++f) g.inside_blocks[f] = get_blocks(r);
   for (int f = 0; f < 4; ++f) g.outside_l_blocks[f] = get_blocks(r);
   for (int f = 0; f < 4; ++f) g.outside_r_blocks[f] = get_blocks(r);
-  g.dec_offsets = r.vec<int32_t>();
-  g.dec_scores = r.vec<float>();
-  g.dec_left = r.vec<int32_t>();
-  g.dec_right = r.vec<int32_t>();
 }
 
 // ---- minimal zip (stored entries) ----
@@ -315,15 +362,59 @@ void save_grz(const std::string& path, const std::vector<const Grammar*>& layers
   if (layers.empty()) throw std::runtime_error("no grammar layers to save");
   std::vector<std::pair<std::string, std::string>> entries;
   std::string names;
-  for (const Grammar* g : layers) names += g->prefix + "\n";
+  for (size_t i = 0; i < layers.size(); ++i) names += (i ? "\n" : "") + std::string("parser-") + std::to_string(i);
   entries.push_back({"parserNames", names});
   for (size_t i = 0; i < layers.size(); ++i) {
     const Grammar& g = *layers[i];
-
     const std::string d = "parser-" + std::to_string(i) + "/";
     entries.push_back({d + "grammar", grammar_payload(g)});
     entries.push_back({d + "structure", structure_payload(g)});
     entries.push_back({d + "semiring", g.semiring == kViterbi ? "Viterbi" : (g.semiring == kReal ? "Real" : "LogSum")});
+    // Rule-kernel bundles (CLInside.scala:16-23, CLOutside.scala:27-38): inside blocks key on the
+    // parent's coarse symbol, outside-left on the left child's, outside-right on the right child's.
+    static const char* kFam[4] = {"NN", "NT", "TN", "TT"};
+    auto coarse_of = [&](int f, int role) {  // role 0 parent, 1 left, 2 right
+      std::vector<int32_t> c;
+      const bool lt = f == kTN || f == kTT, rt = f == kNT || f == kTT;
+      for (const BinaryRule& r : g.binaries[f]) {
+        if (role == 0) c.push_back(g.proj_n[r.p]);
+        else if (role == 1) c.push_back(lt ? g.proj_t[r.l] : g.proj_n[r.l]);
+        else c.push_back(rt ? g.proj_t[r.r] : g.proj_n[r.r]);
+      }
+      return c;
+    };
+    for (int f : {kNT, kNN, kTN, kTT})
+      bundle_entries(entries, d + "inside" + kFam[f], &g.inside_blocks[f], coarse_of(f, 0), g.mask_words);
+    std::vector<int32_t> un(g.unary_n.size(), 0), ut(g.unary_t.size(), 0);
+    bundle_entries(entries, d + "insideNU", nullptr, un, g.mask_words);
+    bundle_entries(entries, d + "insideTU", nullptr, ut, g.mask_words);
+    for (int f : {kNN, kNT, kTN, kTT}) {
+      bundle_entries(entries, d + "outside_L_" + kFam[f], &g.outside_l_blocks[f], coarse_of(f, 1), g.mask_words);
+      bundle_entries(entries, d + "outside_R_" + kFam[f], &g.outside_r_blocks[f], coarse_of(f, 2), g.mask_words);
+    }
+    bundle_entries(entries, d + "outsideNU", nullptr, un, g.mask_words);
+    bundle_entries(entries, d + "outsideTU", nullptr, ut, g.mask_words);
+    // CLParserUtils.write (CLParserUtils.scala:14-16): root scores.
+    kernel_entry(entries, d + "setRootScoresKernel", "k_reduce");
+    kernel_entry(entries, d + "getRootScoresKernel", "k_get_root");
+    // CLMaskKernels.write (CLMaskKernels.scala:23-25): maskSize, blocks per sentence, block size.
+    kernel_entry(entries, d + "computeMasksKernel", "k_masks");
+    entries.push_back({d + "MasksInts", raw(std::vector<int32_t>{g.mask_words, 1, 64})});
+    // CLMBRKernels.write (CLMBRKernels.scala:20-22).
+    kernel_entry(entries, d + "computeMBRKernel", "k_mbr");
+    entries.push_back({d + "MBRInts", raw(std::vector<int32_t>{g.mask_words})});
+    // CLViterbi.write (CLViterbi.scala:73-80): decoder tables, the six group offsets NN, NT, TN,
+    // TT, U, U_T into ViterbiParents.
+    kernel_entry(entries, d + "computeViterbiKernel", "k_viterbi");
+    entries.push_back({d + "ViterbiWGSize", raw(std::vector<int32_t>{1, 256})});
+    entries.push_back({d + "ViterbiParents", raw(g.dec_offsets)});
+    entries.push_back({d + "ViterbiLeft", raw(g.dec_left)});
+    entries.push_back({d + "ViterbiRight", raw(g.dec_right)});
+    entries.push_back({d + "ViterbiScores", raw(g.dec_scores)});
+    const int32_t nN = (int32_t)g.nonterms.size();
+    entries.push_back({d + "ViterbiOffsets", raw(std::vector<int32_t>{0, nN, 2 * nN, 3 * nN, 4 * nN, 5 * nN})});
+    // CLScalingKernels.write (CLScalingKernels.scala:19-20).
+    kernel_entry(entries, d + "getScalingConstantsKernel", "k_scaling");
   }
   write_zip(path, entries);
 }
@@ -335,15 +426,24 @@ std::vector<std::shared_ptr<Grammar>> load_grz(const std::string& path) {
     if (it == z.end()) throw std::runtime_error("bad grz: missing entry " + name);
     return it->second;
   };
+  // CLParserData.readSequence (CLParser.scala:1302-1305): each trimmed line of parserNames names
+  // a layer's entry directory.
   std::istringstream names(get("parserNames"));
   std::vector<std::shared_ptr<Grammar>> out;
   std::string line;
-  for (size_t i = 0; std::getline(names, line); ++i) {
-    if (line.empty()) continue;
-    const std::string d = "parser-" + std::to_string(out.size()) + "/";
+  while (std::getline(names, line)) {
+    const size_t a = line.find_first_not_of(" \t\r"), b = line.find_last_not_of(" \t\r");
+    if (a == std::string::npos) continue;
+    const std::string d = line.substr(a, b - a + 1) + "/";
     auto g = std::make_shared<Grammar>();
     read_grammar(get(d + "grammar"), *g);
     read_structure(get(d + "structure"), *g);
+    g->dec_offsets = unraw<int32_t>(get(d + "ViterbiParents"), "ViterbiParents");
+    g->dec_left = unraw<int32_t>(get(d + "ViterbiLeft"), "ViterbiLeft");
+    g->dec_right = unraw<int32_t>(get(d + "ViterbiRight"), "ViterbiRight");
+    g->dec_scores = unraw<float>(get(d + "ViterbiScores"), "ViterbiScores");
+    if (g->dec_offsets.size() != 6 * g->nonterms.size() + 1 || g->dec_scores.size() != g->dec_left.size())
+      throw std::runtime_error("bad grz payload: Viterbi tables");
     const std::string& sr = get(d + "semiring");
     if (sr == "Viterbi") g->semiring = kViterbi;
     else if (sr == "Real") g->semiring = kReal;

@@ -9,24 +9,52 @@ namespace puck {
 
 Server::Server(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
                int32_t max_inflight, int32_t max_batch, int32_t items_per_lane, bool persistent)
-    : max_batch_(std::max(1, max_batch)) {
+    : max_batch_(std::max(1, max_batch)), device_(device) {
   if (max_inflight < 1) throw std::runtime_error("max_inflight must be >= 1");
   mbr_final_ = layers.back()->semiring != kViterbi;
-  const int64_t per_slot = mem_bytes / max_inflight;
   int32_t max_cell = 0;
   num_words_ = INT32_MAX;
   for (auto& g : layers) {
     max_cell = std::max(max_cell, g->cell_size);
     num_words_ = std::min(num_words_, (int32_t)g->lex_offsets.size() - 1);
   }
-  // Same split as Parser::make_batches: 30% of a slot's memory for two charts per layer.
-  max_cells_ = std::max<int64_t>(1, (int64_t)(per_slot * 0.3) / ((int64_t)max_cell * 4));
+  if (hipSetDevice(device) != hipSuccess) throw std::runtime_error("hipSetDevice failed");
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
+  budget_ = std::min<int64_t>(mem_bytes, (int64_t)free_b - reserve_);
+  if (budget_ <= 0) throw std::runtime_error("no free device memory for the server");
+  // Every slot's parser may take a batch up to the whole budget in one go (its make_batches limit);
+  // admission keeps the slots together within it.
   for (int32_t i = 0; i < max_inflight; ++i) {
-    parsers_.emplace_back(new Parser(layers, device, per_slot, items_per_lane));
+    parsers_.emplace_back(new Parser(layers, device, budget_, items_per_lane));
     parsers_.back()->timing = false;
     parsers_.back()->set_persistent(persistent);
+    parsers_.back()->set_partial_bytes(std::max<int64_t>((int64_t)64 << 20, budget_ / 10 / max_inflight));
   }
+  // A batch never exceeds what one parser keeps in one batch (its chart budget).
+  max_cells_ = std::max<int64_t>(1, (int64_t)(budget_ * 0.6) / 2 / ((int64_t)max_cell * 4));
   for (int32_t i = 0; i < max_inflight; ++i) threads_.emplace_back(&Server::worker, this, i);
+}
+
+// Whether a batch of these lengths fits on `slot`: the bytes beyond what the slot holds must fit into
+// the device's free memory not claimed by other slots, and all slots together into the budget.
+bool Server::fits(int32_t slot, const std::vector<int32_t>& lengths, int64_t* growth) {
+  const int64_t held = parsers_[slot]->workspace_bytes();
+  const int64_t need = parsers_[slot]->bytes_needed(lengths);
+  *growth = std::max<int64_t>(0, need - held);
+  if (*growth == 0) return true;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  int64_t all_held = 0;
+  for (auto& p : parsers_) all_held += p->workspace_bytes();
+  return *growth <= (int64_t)free_b - claimed_ - reserve_ && all_held + claimed_ + *growth <= budget_;
+}
+
+void Server::record_latency(double ms) {
+  if (latencies_ms_.size() < kLatencyWindow) latencies_ms_.push_back(ms);
+  else latencies_ms_[latency_next_] = ms;
+  latency_next_ = (latency_next_ + 1) % kLatencyWindow;
+  ++completed_;
 }
 
 Server::~Server() {
@@ -48,6 +76,7 @@ int64_t Server::submit(const int32_t* words, int32_t len) {
   r.ticket = next_ticket_++;
   r.words.assign(words, words + len);
   r.submitted = std::chrono::steady_clock::now();
+  issued_.insert(r.ticket);
   queue_.push_back(std::move(r));
   ++outstanding_;
   cv_work_.notify_one();
@@ -56,22 +85,31 @@ int64_t Server::submit(const int32_t* words, int32_t len) {
 
 void Server::worker(int32_t slot) {
   Parser& p = *parsers_[slot];
+  if (hipSetDevice(device_) != hipSuccess) return;
   while (true) {
     std::vector<Request> batch;
+    int64_t growth = 0;
     {
       std::unique_lock<std::mutex> lk(mu_);
       cv_work_.wait(lk, [&] { return stop_ || !queue_.empty(); });
       if (stop_ && queue_.empty()) return;
-      // Admit requests in arrival order while they fit this slot's chart budget.
+      // Admit requests in arrival order while the batch fits this slot's memory and the device's
+      // free memory; the first request is always taken (if it cannot be allocated, it fails alone).
       int64_t cells = 0;
+      std::vector<int32_t> lengths;
       while (!queue_.empty() && (int32_t)batch.size() < max_batch_) {
         const int64_t n = (int64_t)queue_.front().words.size();
-        const int64_t c = n * (n + 1);
-        if (!batch.empty() && cells + c > max_cells_) break;
-        cells += c;
+        if (!batch.empty() && cells + n * (n + 1) > max_cells_) break;
+        lengths.push_back((int32_t)n);
+        int64_t g = 0;
+        const bool ok = fits(slot, lengths, &g);
+        if (!ok && !batch.empty()) break;
+        growth = g;
+        cells += n * (n + 1);
         batch.push_back(std::move(queue_.front()));
         queue_.pop_front();
       }
+      claimed_ += growth;
       if (!queue_.empty()) cv_work_.notify_one();
     }
     std::vector<int32_t> words, offsets{0};
@@ -93,6 +131,7 @@ void Server::worker(int32_t slot) {
     const auto now = std::chrono::steady_clock::now();
     {
       std::lock_guard<std::mutex> lk(mu_);
+      claimed_ -= growth;  // allocated by now (or failed)
       for (size_t i = 0; i < batch.size(); ++i) {
         Done d;
         d.status = status;
@@ -102,7 +141,7 @@ void Server::worker(int32_t slot) {
           else d.nodes = std::move(vit[i]);
         }
         done_[batch[i].ticket] = std::move(d);
-        latencies_ms_.push_back(std::chrono::duration<double, std::milli>(now - batch[i].submitted).count());
+        record_latency(std::chrono::duration<double, std::milli>(now - batch[i].submitted).count());
       }
       outstanding_ -= (int64_t)batch.size();
       ++batches_;
@@ -115,6 +154,9 @@ void Server::worker(int32_t slot) {
 bool Server::result(int64_t ticket, bool wait, std::vector<TreeNode>* nodes, std::vector<MbrNode>* mbr,
                     int32_t* status, std::string* error) {
   std::unique_lock<std::mutex> lk(mu_);
+  // Only a ticket that was issued and not yet collected can complete: anything else would wait
+  // forever.
+  if (!issued_.count(ticket)) throw std::runtime_error("unknown or already collected ticket");
   if (wait) {
     cv_done_.wait(lk, [&] { return done_.count(ticket) > 0; });
   } else if (!done_.count(ticket)) {
@@ -122,6 +164,7 @@ bool Server::result(int64_t ticket, bool wait, std::vector<TreeNode>* nodes, std
   }
   Done d = std::move(done_[ticket]);
   done_.erase(ticket);
+  issued_.erase(ticket);
   if (nodes) *nodes = std::move(d.nodes);
   if (mbr) *mbr = std::move(d.mbr);
   if (status) *status = d.status;
@@ -137,7 +180,7 @@ void Server::flush() {
 ServerStats Server::stats() {
   std::lock_guard<std::mutex> lk(mu_);
   ServerStats s;
-  s.completed = (int64_t)latencies_ms_.size();
+  s.completed = completed_;
   s.batches = batches_;
   s.mean_batch = batches_ ? (double)batched_requests_ / batches_ : 0.0;
   if (!latencies_ms_.empty()) {
@@ -155,6 +198,8 @@ ServerStats Server::stats() {
 void Server::reset_stats() {
   std::lock_guard<std::mutex> lk(mu_);
   latencies_ms_.clear();
+  latency_next_ = 0;
+  completed_ = 0;
   batches_ = 0;
   batched_requests_ = 0;
 }

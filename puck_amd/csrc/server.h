@@ -3,8 +3,12 @@
 // Replaces the reference's BatchFunctionAnnotatorService (AnnotatorService.scala:29-78), which
 // accumulates requests and runs one CLParser.parse per flush (20,000 sentences or 1 s). Here
 // requests are admitted into batches as soon as one of `max_inflight` batch slots is free; each slot
-// owns a Parser with its own HIP stream and workspace, so batches overlap on the device, and a
-// slot admits requests while their chart cells fit its memory budget.
+// owns a Parser with its own HIP stream and workspace, so batches overlap on the device. Admission
+// is by free device memory: a slot takes queued requests in arrival order while the memory its batch
+// needs (Parser::bytes_needed) fits into what the slot already holds plus the device's free memory
+// (hipMemGetInfo, less what other slots have claimed but not yet allocated and a reserve), and the
+// whole server stays within its memory budget (WorkSpace.allocate's rule, WorkSpace.scala:88-122,
+// applied per admission instead of once per call).
 #pragma once
 
 #include <atomic>
@@ -16,6 +20,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "engine.h"
@@ -71,8 +76,19 @@ class Server {
   int64_t next_ticket_ = 1;
   int64_t outstanding_ = 0;
   bool stop_ = false;
+  // Latencies of the last kLatencyWindow completed requests (a ring).
+  static constexpr size_t kLatencyWindow = 1 << 20;
   std::vector<double> latencies_ms_;
+  size_t latency_next_ = 0;
+  int64_t completed_ = 0;
   int64_t batches_ = 0, batched_requests_ = 0;
+  std::unordered_set<int64_t> issued_;  // tickets submitted and not yet collected
+  int64_t budget_ = 0;                  // device bytes all slots together may hold
+  int64_t reserve_ = (int64_t)256 << 20;
+  int64_t claimed_ = 0;                 // growth admitted by slots but not yet allocated
+  int32_t device_ = 0;
+  bool fits(int32_t slot, const std::vector<int32_t>& lengths, int64_t* growth);
+  void record_latency(double ms);
 };
 
 }  // namespace puck

@@ -65,3 +65,20 @@ def test_server_rejects_bad_requests_at_submit(puck, wsj1):
     srv.flush()
     for i, s in enumerate(sents):
         assert np.array_equal(srv.result(tickets[i], len(s)), want[i]), f"request {i}"
+
+
+def test_server_ticket_errors_and_tight_memory_budget(puck, wsj1):
+    """A ticket that was never issued, or was already collected, is an error (not a wait). Under a
+    tight memory budget the slots admit smaller batches and every result is unchanged."""
+    words, offsets = wsj1.sample(seed=80, n=40, mode=0, min_len=5, max_len=30)
+    want = puck.CLParser([wsj1], mem_bytes=1 << 30).parse((words, offsets))
+    srv = puck.Server([wsj1], mem_bytes=384 << 20, max_inflight=2)
+    sents = split_sentences(words, offsets)
+    tickets = [srv.submit(s) for s in sents]
+    with pytest.raises(puck.PuckError, match="unknown or already collected"):
+        srv.result(10 ** 9, 3)
+    for i, s in enumerate(sents):
+        assert np.array_equal(srv.result(tickets[i], len(s)), want[i]), f"request {i}"
+    with pytest.raises(puck.PuckError, match="unknown or already collected"):
+        srv.result(tickets[0], len(sents[0]))
+    assert srv.stats()["completed"] == len(sents)

@@ -13,17 +13,53 @@ def grz(tmp_path_factory, puck, wsj0, wsj1):
     return path
 
 
-def test_zip_layout_and_entry_names(grz, puck):
+# Entry names the reference writes per layer (CLParser.scala:1249-1260 and the write methods it
+# calls: CLInside.scala:16-23, CLOutside.scala:27-38, CLParserUtils.scala:14-16,
+# CLMaskKernels.scala:23-25, CLMBRKernels.scala:20-22, CLViterbi.scala:73-80,
+# CLScalingKernels.scala:19-20; a kernel is `<name>.program` + `<name>.name`, ZipUtil.scala:86-89).
+REFERENCE_LAYER_ENTRIES = (
+    ["grammar", "structure", "semiring"]
+    + [f"{b}/numKernels" for b in ("insideNT", "insideNN", "insideTN", "insideTT", "insideNU", "insideTU",
+                                   "outside_L_NN", "outside_R_NN", "outside_L_NT", "outside_R_NT", "outside_L_TN",
+                                   "outside_R_TN", "outside_L_TT", "outside_R_TT", "outsideNU", "outsideTU")]
+    + [f"{b}/{e}" for b in ("insideNN", "outside_L_NN") for e in ("directWrite", "extra", "enqueuer/ints",
+                                                                 "enqueuer/enqueue.name", "0/bits", "0/rules")]
+    + [f"{k}.{x}" for k in ("setRootScoresKernel", "getRootScoresKernel", "computeMasksKernel", "computeMBRKernel",
+                            "computeViterbiKernel", "getScalingConstantsKernel") for x in ("program", "name")]
+    + ["MasksInts", "MBRInts", "ViterbiWGSize", "ViterbiParents", "ViterbiLeft", "ViterbiRight", "ViterbiScores",
+       "ViterbiOffsets"])
+
+
+def test_zip_layout_and_entry_names(grz, puck, wsj0, wsj1):
     with zipfile.ZipFile(grz) as z:
-        names = z.namelist()
-        assert names[0] == "parserNames"
-        for i in range(2):
-            for e in ("grammar", "structure", "semiring"):
-                assert f"parser-{i}/{e}" in names
+        names = set(z.namelist())
+        assert z.namelist()[0] == "parserNames"
+        # CLParserData.writeSequence (CLParser.scala:1307-1310): "parser-i" lines.
+        assert z.read("parserNames").decode().split("\n") == ["parser-0", "parser-1"]
+        for i, g in enumerate((wsj0, wsj1)):
+            for e in REFERENCE_LAYER_ENTRIES:
+                assert f"parser-{i}/{e}" in names, e
+            # one rule-kernel entry per CoarseParent block of the inside families
+            for f, fam in enumerate(("NN", "NT", "TN", "TT")):
+                assert z.read(f"parser-{i}/inside{fam}/numKernels") == str(g.info.num_blocks[f]).encode()
+            n_nn = np.frombuffer(z.read(f"parser-{i}/insideNN/0/rules"), np.int32)
+            assert n_nn.size > 0 and np.all(np.diff(n_nn) > 0)
+            off = np.frombuffer(z.read(f"parser-{i}/ViterbiOffsets"), np.int32)
+            assert list(off) == [k * g.info.num_nonterms for k in range(6)]
         assert z.testzip() is None  # CRCs agree
-        lines = z.read("parserNames").decode().split()
-        assert lines == [puck.grammar_path("wsj_0"), puck.grammar_path("wsj_1")]
         assert z.read("parser-1/semiring") == b"Viterbi"
+        assert z.read("parser-0/computeViterbiKernel.name") == b"k_viterbi"
+
+
+def test_block_rule_lists_partition_each_family(grz, wsj1):
+    """The per-block rule lists of a bundle partition the family's rules."""
+    with zipfile.ZipFile(grz) as z:
+        for fam in ("NN", "NT", "TN", "TT"):
+            nb = int(z.read(f"parser-1/inside{fam}/numKernels"))
+            rules = np.concatenate([np.frombuffer(z.read(f"parser-1/inside{fam}/{j}/rules"), np.int32)
+                                    for j in range(nb)])
+            n = len(wsj1.rules(("NN", "NT", "TN", "TT").index(fam))[0])
+            assert sorted(rules.tolist()) == list(range(n))
 
 
 def test_tables_round_trip(grz, puck, wsj0, wsj1):
