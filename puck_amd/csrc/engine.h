@@ -120,6 +120,9 @@ class Layer {
   double* d_lex_logp = nullptr;
   int32_t* d_proj_n = nullptr;
   int32_t* d_proj_t = nullptr;
+  // MBR: per coarse symbol the symbols projecting to it (offsets num_coarse + 1, symbol ids), for
+  // the terminal (span-1 bottom) and nonterminal projections.
+  int32_t *d_mbr_t_off = nullptr, *d_mbr_t_syms = nullptr, *d_mbr_n_off = nullptr, *d_mbr_n_syms = nullptr;
 
  private:
   void build_family(int32_t id);

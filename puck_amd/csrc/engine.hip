@@ -1046,23 +1046,33 @@ __global__ void k_masks(uint32_t* masks_out, const float* inside, const float* o
   }
 }
 
-// getScalingConstants (CLScalingKernels.scala:51-69).
-__global__ void k_scaling(float* scale, const float* chart, int32_t num_syms, int32_t num_cells) {
-  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+// getScalingConstants (CLScalingKernels.scala:51-69): scale = max over the cell's symbols (start
+// -100000), 0 below -9000. One wave per cell: lanes read consecutive symbols (coalesced) and the
+// maximum is combined across the wave; max is exact, so the order does not matter.
+__global__ __launch_bounds__(256) void k_scaling(float* scale, const float* chart, int32_t num_syms, int32_t num_cells) {
+  const int cell = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (cell >= num_cells) return;
   float m = -100000.0f;
   const float* c = chart + (size_t)cell * num_syms;
-  for (int sym = 0; sym < num_syms; ++sym) m = fmaxf(c[sym], m);
-  if (m < -9000.0f) m = 0;
-  scale[cell] = m;
+  for (int sym = lane; sym < num_syms; sym += 64) m = fmaxf(c[sym], m);
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (lane == 0) scale[cell] = m < -9000.0f ? 0.0f : m;
 }
 
-// computeMBR (CLMBRKernels.scala:118-171): one thread per cell, symbols in order.
-__global__ void k_mbr(MbrCell* out, const float* inside, const float* outside,
-                      const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
-                      int32_t num_cells, const int32_t* proj_t, const int32_t* proj_n,
-                      int32_t num_syms, int32_t cell_size, int32_t root, int32_t num_coarse, int32_t logsum) {
-  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+// computeMBR (CLMBRKernels.scala:118-171): per cell, the coarse marginals
+// margs[c] = sum over the cell's symbols of coarse c of in / root * out (LogSum: exp(in - root + out),
+// CLMBRKernels.scala:145-150), then the coarse symbol with the largest marginal, the first one on
+// ties. One wave per cell: lane c sums coarse c's symbols in increasing symbol order (the order in
+// which the reference's loop over symbols adds into margs[c]; symbols past the half's own space count
+// as coarse 0, as their projection is padded with 0), and the wave picks the best (score, coarse).
+__global__ __launch_bounds__(256) void k_mbr(MbrCell* out, const float* inside, const float* outside,
+                                             const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
+                                             int32_t num_cells, const int32_t* t_off, const int32_t* t_syms,
+                                             const int32_t* n_off, const int32_t* n_syms, int32_t cell_size,
+                                             int32_t root, int32_t num_coarse, int32_t logsum) {
+  const int cell = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (cell >= num_cells) return;
   int lo = 0, hi = num_sent;
   while (hi - lo > 1) {
@@ -1070,26 +1080,35 @@ __global__ void k_mbr(MbrCell* out, const float* inside, const float* outside,
     if (cell_offsets[mid] <= cell) lo = mid; else hi = mid;
   }
   const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
-  const int n = lengths[sent];
+  (void)lengths;
   const float root_score = inside[(size_t)(last - 1) * cell_size + root];
-  const int32_t* proj = (cell - first) < n ? proj_t : proj_n;
+  const bool term = (cell - first) < lengths[sent];
+  const int32_t* off = term ? t_off : n_off;
+  const int32_t* syms = term ? t_syms : n_syms;
   const float* in = inside + (size_t)cell * cell_size;
   const float* o = outside + (size_t)cell * cell_size;
-  float margs[kMaxCoarseSymbols];
-  for (int c = 0; c < num_coarse; ++c) margs[c] = 0.0f;
-  for (int sym = 0; sym < num_syms; ++sym) {
-    // CLMBRKernels.scala:145-150: exp(in - root + out) under LogSum, in / root * out otherwise.
-    const float v = logsum ? expf(in[sym] - root_score + o[sym]) : in[sym] / root_score * o[sym];
-    margs[proj[sym]] += v;
-  }
-  MbrCell d{-INFINITY, -1};
-  for (int c = 0; c < num_coarse; ++c) {
-    if (margs[c] > d.score) {
-      d.score = margs[c];
-      d.symbol = c;
+  float best = -INFINITY;
+  int best_c = -1;
+  for (int c = lane; c < num_coarse; c += 64) {
+    float m = 0.0f;
+    for (int k = off[c]; k < off[c + 1]; ++k) {
+      const int sym = syms[k];
+      m += logsum ? expf(in[sym] - root_score + o[sym]) : in[sym] / root_score * o[sym];
+    }
+    if (m > best) {  // lanes visit their coarse symbols in increasing order: first max wins
+      best = m;
+      best_c = c;
     }
   }
-  out[cell] = d;
+  for (int w = 32; w > 0; w >>= 1) {
+    const float ob = __shfl_xor(best, w);
+    const int oc = __shfl_xor(best_c, w);
+    if (ob > best || (ob == best && oc >= 0 && (best_c < 0 || oc < best_c))) {
+      best = ob;
+      best_c = oc;
+    }
+  }
+  if (lane == 0) out[cell] = MbrCell{best, best_c};
 }
 
 // viterbi (CLViterbi.scala:495-605): top-down best tree, one wave per sentence. The reference scans
@@ -1359,6 +1378,24 @@ Layer::Layer(Grammar gr, bool final_l, bool masked_l, int32_t items_per_lane)
   pt.resize(g.num_syms, 0);
   d_proj_n = upload(pn);
   d_proj_t = upload(pt);
+  // MBR symbol lists: per coarse symbol, the symbols projecting to it in increasing order, over the
+  // padded projection (symbols past a half's own space project to 0).
+  const int32_t nc = (int32_t)g.coarse_names.size();
+  for (int half = 0; half < 2; ++half) {
+    const std::vector<int32_t>& pr = half ? pn : pt;
+    std::vector<int32_t> off(nc + 1, 0), syms(g.num_syms);
+    for (int32_t sym = 0; sym < g.num_syms; ++sym) ++off[pr[sym] + 1];
+    for (int32_t c = 0; c < nc; ++c) off[c + 1] += off[c];
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int32_t sym = 0; sym < g.num_syms; ++sym) syms[fill[pr[sym]]++] = sym;
+    if (half) {
+      d_mbr_n_off = upload(off);
+      d_mbr_n_syms = upload(syms);
+    } else {
+      d_mbr_t_off = upload(off);
+      d_mbr_t_syms = upload(syms);
+    }
+  }
 }
 
 Layer::~Layer() {
@@ -2697,9 +2734,9 @@ void Parser::launch_masks_and_scales(Batch& B, int32_t li, int32_t mask_words) {
                      g.cell_size, g.root, -9.0f, mask_words);
   HIP_CHECK(hipGetLastError());
   if (layers_[li + 1]->g.semiring == kReal) {
-    hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+    hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_,
                        (float*)in_scale_.p, (const float*)inside_.p, g.cell_size, B.num_cells);
-    hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+    hipLaunchKernelGGL(k_scaling, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_,
                        (float*)out_scale_.p, (const float*)outside_.p, g.cell_size, B.num_cells);
     HIP_CHECK(hipGetLastError());
     B.has_scales = true;
@@ -2723,10 +2760,11 @@ void Parser::launch_viterbi(Batch& B, int32_t li) {
 void Parser::launch_mbr(Batch& B, int32_t li) {
   const Layer& L = *layers_[li];
   const Grammar& g = L.g;
-  hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 127) / 128), dim3(128), 0, stream_,
+  hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_,
                      (MbrCell*)mbr_.p, (const float*)inside_.p, (const float*)outside_.p,
-                     B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_proj_t, L.d_proj_n,
-                     g.num_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size(), g.semiring == kLogSum);
+                     B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_mbr_t_off, L.d_mbr_t_syms,
+                     L.d_mbr_n_off, L.d_mbr_n_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size(),
+                     g.semiring == kLogSum);
   HIP_CHECK(hipGetLastError());
 }
 
