@@ -160,6 +160,15 @@ int32_t puck_parse_viterbi(puck_parser_t* p, const int32_t* words, const int32_t
                            puck_tree_node_t* nodes, int64_t nodes_cap);
 int32_t puck_parse_mbr(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
                        puck_mbr_node_t* nodes, int64_t nodes_cap);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): puck_parse_viterbi's result
+ * with no device allocation. Every workspace buffer (charts, scales, masks, mask plans, the rule
+ * kernels' partial rows and counters) is carved from a static per-device array of 256 MiB that the
+ * library holds until the process ends; an input whose workspace would exceed it is rejected with
+ * PUCK_ERR_ARG before any device work. Calls that overlap on one GPU share that array and race (the
+ * handle's own lock serialises calls on one handle only). Viterbi final layers only. */
+int32_t puck_parse_viterbi_noalloc(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
+                                   puck_tree_node_t* nodes, int64_t nodes_cap);
 int32_t puck_partitions(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n,
                         float* out);
 int32_t puck_inside_outside(puck_parser_t* p, const int32_t* words, const int32_t* offsets, int32_t n);

@@ -116,6 +116,8 @@ SIGNATURES = [
     ("puck_batch_stats", ctypes.c_int32, [_P, ctypes.POINTER(BatchStats)]),
     ("puck_parse_viterbi", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(TreeNode), ctypes.c_int64]),
     ("puck_parse_mbr", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(MbrNode), ctypes.c_int64]),
+    ("puck_parse_viterbi_noalloc", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, ctypes.POINTER(TreeNode),
+                                                    ctypes.c_int64]),
     ("puck_partitions", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, _F32P]),
     ("puck_inside_outside", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32]),
     ("puck_masks", ctypes.c_int32, [_P, _I32P, _I32P, ctypes.c_int32, _U32P, ctypes.c_int64]),

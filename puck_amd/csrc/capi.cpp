@@ -339,6 +339,23 @@ int32_t puck_parse_viterbi(puck_parser_t* h, const int32_t* words, const int32_t
   });
 }
 
+int32_t puck_parse_viterbi_noalloc(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
+                                   puck_tree_node_t* nodes, int64_t cap) {
+  return guarded([&]() -> int32_t {
+    if (!h || !words || !offsets || !nodes || n < 0) return set_error(PUCK_ERR_ARG, "bad argument");
+    if (nodes_needed(offsets, n) > cap) return set_error(PUCK_ERR_ARG, "node buffer too small");
+    std::vector<std::vector<puck::TreeNode>> trees;
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->p->parse_noalloc(words, offsets, n, &trees, nullptr);
+    int64_t o = 0;
+    for (auto& t : trees) {
+      std::memcpy(nodes + o, t.data(), t.size() * sizeof(puck_tree_node_t));
+      o += (int64_t)t.size();
+    }
+    return PUCK_OK;
+  });
+}
+
 int32_t puck_parse_mbr(puck_parser_t* h, const int32_t* words, const int32_t* offsets, int32_t n,
                        puck_mbr_node_t* nodes, int64_t cap) {
   return guarded([&]() -> int32_t {

@@ -157,6 +157,15 @@ class Parser {
   // CLParser.parse: Viterbi trees (final layer Viterbi) or MBR trees (final layer Real).
   void parse(const int32_t* words, const int32_t* offsets, int32_t n,
              std::vector<std::vector<TreeNode>>* viterbi, std::vector<std::vector<MbrNode>>* mbr);
+  // parse() without any device allocation: every workspace buffer is carved from the library's
+  // file-scope scratch array on this device (kNoAllocScratchBytes; 1/4 of it holds the rule kernels'
+  // partial rows). Inputs whose workspace would not fit are rejected before any device work. No
+  // graph capture, no timing events beyond a pool made when the parser was created. Calls that
+  // overlap on one GPU share that GPU's array and race; the kernels clear what they need at the start
+  // of every batch (fill kernels), so a call never depends on what the previous one left behind.
+  static constexpr int64_t kNoAllocScratchBytes = (int64_t)256 << 20;
+  void parse_noalloc(const int32_t* words, const int32_t* offsets, int32_t n,
+                     std::vector<std::vector<TreeNode>>* viterbi, std::vector<std::vector<MbrNode>>* mbr);
   // CLParser.partitions: sentence inside log-likelihood under the final layer.
   void partitions(const int32_t* words, const int32_t* offsets, int32_t n, float* out);
   // CLParser.insideOutside: inside, outside and the decode kernels, results kept on device.
@@ -184,7 +193,8 @@ class Parser {
   // Device bytes one call on sentences of these lengths makes the workspace hold (an upper bound
   // that counts every masked item as planned): charts, scales, masks, mask plans, the partial
   // buffer and the batch tables. Serving admits requests by it.
-  int64_t bytes_needed(const std::vector<int32_t>& lengths) const;
+  int64_t bytes_needed(const std::vector<int32_t>& lengths) const { return bytes_needed(lengths, 0); }
+  int64_t bytes_needed(const std::vector<int32_t>& lengths, int64_t partial_bytes) const;
   // When set, each batch's device interval is also measured from this event (recorded on any
   // stream of the same device before the call): stats.span_lo_ms / span_hi_ms.
   hipEvent_t ref_event = nullptr;
@@ -270,6 +280,12 @@ class Parser {
   Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
       pos_scale_, aux_, mplan_, apps_, queue_, jobs_;
   int32_t queue_next_ = 0;
+  // No-alloc mode: ensure() carves buffers from [base, base + size) instead of allocating.
+  struct Arena { char* base = nullptr; size_t size = 0, used = 0; };
+  Arena arena_;
+  char* scratch_ = nullptr;  // this device's scratch array (device address), set at construction
+  std::string scratch_err_;  // why scratch_ is null, if it is
+  void reset_arena();
   int32_t num_cus_ = 256;
   void ensure(Buf& b, size_t bytes);
 

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -32,6 +34,11 @@ namespace puck {
       throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
                                #x);                                                          \
   } while (0)
+
+// Device scratch of Parser::parse_noalloc (see engine.h): one array per GPU, held until the process
+// ends. Host code never passes `scratch` itself to a kernel: the array's device address on the
+// current device comes from hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) unsigned char scratch[Parser::kNoAllocScratchBytes];
 
 namespace {
 
@@ -1756,11 +1763,40 @@ struct Parser::Batch {
   bool has_scales = false;
 };
 
+namespace {
+std::mutex g_scratch_mu;
+std::map<int32_t, char*> g_scratch_ptr;
+
+std::map<int32_t, std::string> g_scratch_err;
+
+// Device address of the scratch array on `device`, looked up once per device with that device
+// current; nullptr (and the reason in *err) if the runtime cannot give it.
+char* scratch_address(int32_t device, std::string* err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *err = g_scratch_err[device];
+    return it->second;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    (void)hipGetLastError();
+    g_scratch_err[device] = std::string("HIP error: ") + hipGetErrorString(e) + " at hipGetSymbolAddress(scratch)";
+    p = nullptr;
+  }
+  g_scratch_ptr[device] = static_cast<char*>(p);
+  *err = g_scratch_err[device];
+  return static_cast<char*>(p);
+}
+}  // namespace
+
 Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int64_t mem_bytes,
                int32_t items_per_lane)
     : device_(device), mem_bytes_(mem_bytes) {
   if (layers.empty()) throw std::runtime_error("parser needs at least one grammar layer");
   HIP_CHECK(hipSetDevice(device_));
+  scratch_ = scratch_address(device_, &scratch_err_);
   for (size_t i = 0; i < layers.size(); ++i) {
     if (i + 1 < layers.size() && layers[i]->semiring != kViterbi)
       throw std::runtime_error("coarse layers must use the Viterbi semiring");
@@ -1774,6 +1810,12 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_CHECK(hipEventCreate(&ev0_));
   HIP_CHECK(hipEventCreate(&ev1_));
+  // Event pool: the no-alloc parse uses these and creates none.
+  for (int i = 0; i < 64; ++i) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+  }
   // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
   const int lds = kLdsBudgetBytes;
   for (const void* fn : {(const void*)k_rules<2, kViterbi, false, false>, (const void*)k_rules<2, kViterbi, true, false>,
@@ -1835,6 +1877,17 @@ Parser::GraphEntry* Parser::graph_entry(const std::vector<int32_t>& lengths) {
 void Parser::ensure(Buf& b, size_t bytes) {
   bytes = std::max<size_t>(bytes, 256);
   if (b.bytes >= bytes) return;
+  if (arena_.base != nullptr) {
+    // No-alloc mode: carve from the scratch array (256-byte aligned, bump order).
+    const size_t off = (arena_.used + 255) & ~(size_t)255;
+    if (off + bytes > arena_.size)
+      throw std::runtime_error("the no-alloc parse needs more than its " + std::to_string(arena_.size) +
+                               " bytes of device scratch for this input");
+    b.p = arena_.base + off;
+    b.bytes = bytes;
+    arena_.used = off + bytes;
+    return;
+  }
   // A graph being captured cannot take new memory; graphs captured before hold the old address.
   if (capturing_) throw CaptureAbort();
   ++ws_generation_;
@@ -1853,7 +1906,7 @@ int64_t Parser::workspace_bytes() const {
   return total;
 }
 
-int64_t Parser::bytes_needed(const std::vector<int32_t>& lengths) const {
+int64_t Parser::bytes_needed(const std::vector<int32_t>& lengths, int64_t partial_bytes) const {
   int32_t max_cell = 0;
   for (auto& l : layers_) max_cell = std::max(max_cell, l->g.cell_size);
   const int64_t mw = layers_[0]->g.mask_words;
@@ -1872,8 +1925,9 @@ int64_t Parser::bytes_needed(const std::vector<int32_t>& lengths) const {
       plan_ints += (int64_t)(layers_.size() - 1) * (9 * 16 * tri_n + in_items + out_items);
     }
   }
-  const int64_t partial = partial_bytes_ > 0 ? partial_bytes_
-                                            : std::max<int64_t>((int64_t)(mem_bytes_ * 0.3), (int64_t)64 << 20);
+  const int64_t partial = partial_bytes > 0    ? partial_bytes
+                          : partial_bytes_ > 0 ? partial_bytes_
+                                               : std::max<int64_t>((int64_t)(mem_bytes_ * 0.3), (int64_t)64 << 20);
   return cells * (2 * (int64_t)max_cell * 4 + 8 + 8 * mw + (int64_t)sizeof(MbrCell)) + words * 8 + 4 * plan_ints +
          4 * tables + partial + ((int64_t)16 << 20);
 }
@@ -2118,6 +2172,7 @@ void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids
 
 hipEvent_t Parser::next_event() {
   if (ev_next_ == ev_pool_.size()) {
+    if (arena_.base != nullptr) throw std::runtime_error("event pool exhausted in the no-alloc parse");
     hipEvent_t e;
     HIP_CHECK(hipEventCreate(&e));
     ev_pool_.push_back(e);
@@ -2535,6 +2590,7 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
                        int32_t what, std::vector<std::vector<TreeNode>>* viterbi,
                        std::vector<std::vector<MbrNode>>* mbr, float* partitions, Charts* charts,
                        int32_t chart_layer, std::vector<uint32_t>* masks_out) {
+  if (arena_.base != nullptr) reset_arena();  // the previous batch has finished with its buffers
   Batch B;
   B.n = count;
   B.cell_offsets.push_back(0);
@@ -3018,6 +3074,62 @@ void Parser::parse(const int32_t* words, const int32_t* offsets, int32_t n,
   if (layers_.back()->g.semiring == kViterbi && !viterbi) throw std::runtime_error("viterbi output required");
   if (layers_.back()->g.semiring != kViterbi && !mbr) throw std::runtime_error("mbr output required");
   for (auto [f, c] : make_batches(offsets, n)) run_batch(words, offsets, f, c, 0, viterbi, mbr, nullptr, nullptr, 0, nullptr);
+}
+
+void Parser::reset_arena() {
+  for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_, &trees_, &mbr_,
+                 &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_})
+    *b = Buf{};
+  arena_.used = 0;
+}
+
+void Parser::parse_noalloc(const int32_t* words, const int32_t* offsets, int32_t n,
+                           std::vector<std::vector<TreeNode>>* viterbi, std::vector<std::vector<MbrNode>>* mbr) {
+  HIP_CHECK(hipSetDevice(device_));
+  if (scratch_ == nullptr) throw std::runtime_error("no scratch array on this device: " + scratch_err_);
+  std::vector<int32_t> lens;
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t len = offsets[i + 1] - offsets[i];
+    if (len < 1 || len > kMaxLength) throw std::runtime_error("sentence length must be in [1, 255]");
+    lens.push_back(len);
+  }
+  const int64_t partial = kNoAllocScratchBytes / 4;
+  const int64_t need = bytes_needed(lens, partial);
+  if (need > kNoAllocScratchBytes)
+    throw std::runtime_error("input needs about " + std::to_string(need) + " bytes of device workspace; the no-alloc "
+                             "parse holds at most " + std::to_string(kNoAllocScratchBytes));
+  // Swap the parser's own workspace out for the scratch array, and back afterwards.
+  Buf* bufs[] = {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_, &trees_, &mbr_,
+                 &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_};
+  std::vector<Buf> saved;
+  for (Buf* b : bufs) saved.push_back(*b);
+  const bool saved_graphs = use_graphs, saved_timing = timing;
+  const int64_t saved_partial = partial_bytes_;
+  const int64_t saved_mem = mem_bytes_;
+  use_graphs = false;
+  timing = false;
+  partial_bytes_ = partial;
+  mem_bytes_ = kNoAllocScratchBytes * 10;  // one batch: the size check above already bounds the input
+  arena_ = Arena{scratch_, (size_t)kNoAllocScratchBytes, 0};
+  reset_arena();
+  ++ws_generation_;
+  auto restore = [&]() {
+    for (size_t k = 0; k < saved.size(); ++k) *bufs[k] = saved[k];
+    use_graphs = saved_graphs;
+    timing = saved_timing;
+    partial_bytes_ = saved_partial;
+    mem_bytes_ = saved_mem;
+    arena_ = Arena{};
+    ++ws_generation_;
+  };
+  try {
+    parse(words, offsets, n, viterbi, mbr);
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    restore();
+    throw;
+  }
+  restore();
 }
 
 void Parser::partitions(const int32_t* words, const int32_t* offsets, int32_t n, float* out) {
