@@ -200,18 +200,26 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
         k = (code >> 8) - a.slice_cell0;
         j = code & 255;
       } else if (a.prefix == nullptr && a.pair_splits) {
-        // Paired split-major items: items 2m and 2m+1 (one lane) are splits 2p and 2p+1 of one
-        // cell, pair m = p * num_cells + cell; a lane folds its pair before writing the partial.
+        // Paired block-major items: pair m = (block of 64 cells, split pair p, cell of the block);
+        // items 2m and 2m+1 (one lane) are splits 2p and 2p+1 of that cell, and the lane folds its
+        // pair before writing the partial.
         const int m = item >> 1;
-        const int pr = m / a.num_cells;
-        k = m - pr * a.num_cells;
-        j = 2 * pr + (item & 1);
-        valid = j < a.span - 1;
+        const int per_block = 64 * a.uniform_count;
+        const int bx = m / per_block;
+        const int r = m - bx * per_block;
+        k = bx * 64 + (r & 63);
+        j = 2 * (r >> 6) + (item & 1);
+        valid = j < a.span - 1 && k < a.num_cells;
       } else if (a.prefix == nullptr) {
-        // Uniform families are split-major: item = split * num_cells + cell, so that the reduce
-        // reads each split's partials of consecutive cells as one coalesced row.
-        j = item / a.num_cells;
-        k = item - j * a.num_cells;
+        // Uniform families are block-major: item = (block of 64 cells, split, cell of the block),
+        // so that a tile lies in one or two blocks and the reduce reads a block's split rows of
+        // consecutive cells coalesced. The last block's rows are padded to 64 cells.
+        const int per_block = 64 * a.uniform_count;
+        const int bx = item / per_block;
+        const int r = item - bx * per_block;
+        j = r >> 6;
+        k = bx * 64 + (r & 63);
+        valid = k < a.num_cells;
       } else {
         const int abs_item = item + item_begin;
         int lo = 0, hi = a.num_cells;  // largest k with prefix[k] <= abs_item
@@ -891,8 +899,8 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
         const float* part = a.partial[f];
         const size_t pitch = (size_t)a.pitch[f];
         if (a.prefix[f] == nullptr) {
-          // Split-major items: split j of cell k is item j * num_cells + k; lanes read consecutive
-          // cells of one split row.
+          // Block-major items: split row j of this block's 64 cells is item (block * uc + j) * 64 +
+          // cell of the block; lanes read consecutive cells of one split row.
           const int uc = a.uniform_count[f];
           const int total = (kr - k) * uc;
           const bool live = my_k < kend && use;
@@ -904,7 +912,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
               v[u] = semi_zero<SEMI>();
               if (m + u < total) {
                 const int sl = __builtin_amdgcn_readlane(ls, ent);
-                if (live) v[u] = part[(size_t)sl * pitch + (size_t)jj * a.num_cells + my_k];
+                if (live) v[u] = part[(size_t)sl * pitch + ((size_t)blockIdx.x * uc + jj) * 64 + lane];
                 if (++jj == uc) {
                   jj = 0;
                   ++ent;
@@ -1972,6 +1980,9 @@ void Parser::timed_end(double* acc) {
 
 namespace {
 bool uniform_kind(int kind) { return kind <= 2 || kind == 7; }
+// Items of a block-major uniform family over kc cells with `rows` rows (splits or split pairs) per
+// cell: whole 64-cell blocks.
+int64_t uniform_items(int64_t kc, int64_t rows) { return (kc + 63) / 64 * 64 * rows; }
 // Unplanned inside NN launches with two items per lane fold split pairs in the rule kernel when
 // the splits pair up without a lone last split or that lone split is a small share of the work.
 bool pairs_splits(const Layer& L, const DevFamily& F, int span) {
@@ -2020,12 +2031,12 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   } else if (uniform_kind(F.kind)) {
     a.prefix = nullptr;
     a.uniform_count = F.kind == 0 ? span - 1 : 1;
-    nitems = (int64_t)kc * a.uniform_count;
+    nitems = uniform_items(kc, a.uniform_count);
     if (pairs_splits(L, F, span)) {
       // Items 2m, 2m+1 are splits 2p, 2p+1 of one cell; a partial row holds one value per pair.
       a.pair_splits = 1;
       a.uniform_count = span / 2;  // pairs: ceil((span - 1) / 2)
-      nitems = 2 * (int64_t)kc * a.uniform_count;
+      nitems = 2 * uniform_items(kc, a.uniform_count);
     }
   } else {
     const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
@@ -2243,7 +2254,12 @@ std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t 
   std::vector<std::pair<int32_t, int32_t>> out;
   const int64_t budget_floats = (int64_t)(partial_.bytes / 4);
   int64_t fixed = 0;
-  for (int fid : R.families) fixed += 256 * (int64_t)L.fam[fid].num_slots;
+  // Row padding: pitches round up to 256 values, block-major uniform rows to whole 64-cell blocks.
+  for (int fid : R.families) {
+    const DevFamily& F = L.fam[fid];
+    const int64_t rows = uniform_kind(F.kind) ? (F.kind == 0 ? std::max(span - 1, 1) : 1) : 0;
+    fixed += (256 + 64 * rows) * (int64_t)F.num_slots;
+  }
   int32_t k0 = 0;
   while (k0 < ncells) {
     int32_t k1 = k0;
@@ -2475,9 +2491,10 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
         ra.uniform_count[fi] = 1;
       }
       int64_t row_items = nitems;  // partial values per slot row
+      if (!use_masks && uniform_kind(F.kind)) row_items = uniform_items(kc, ra.uniform_count[fi]);
       if (!use_masks && pairs_splits(L, F, span)) {
         ra.uniform_count[fi] = span / 2;
-        row_items = (int64_t)kc * (span / 2);
+        row_items = uniform_items(kc, span / 2);
       }
       const int32_t pitch = (int32_t)((row_items + 255) / 256 * 256);
       ra.partial[fi] = base + off;
