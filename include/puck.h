@@ -136,9 +136,16 @@ enum puck_parser_flags {
                                       graph on its second sighting and replay it afterwards */
   PUCK_FLAG_NO_PLAN_AHEAD = 8,     /* accepted and ignored: every parser issues all of its device work,
                                       mask plans included, on its one HIP stream */
-  PUCK_FLAG_OUTSIDE_FINAL = 16     /* puck_parse_viterbi also runs the final layer's outside pass before
+  PUCK_FLAG_OUTSIDE_FINAL = 16,    /* puck_parse_viterbi also runs the final layer's outside pass before
                                       the decode, as CLParser.insideOutside does
                                       (CLParser.scala:171-182); the trees do not change */
+  PUCK_FLAG_DEVICE_SCHEDULE = 32   /* each pass runs as one persistent launch that pulls the rule and
+                                      reduce work of all its spans from a device queue, a span of a
+                                      sentence starting once that sentence's shorter spans are done
+                                      (per-block completion counters); would replace the span loop of
+                                      CLParser.scala:284-291,331-342 on the device. DISABLED: rejected
+                                      with PUCK_ERR_ARG (the launch did not finish on the GPU and no
+                                      cause has been shown) */
 };
 /* new CLParser(data, maxAllocSize) (CLParser.scala:46-48): mem_bytes bounds the device workspace,
  * as the reference's --mem option bounds WorkSpace.allocate (WorkSpace.scala:91-120). */
@@ -152,6 +159,8 @@ typedef struct {
   int32_t persistent;
   int32_t piece_target;     /* >= 1, default 512 */
   int32_t queue_wgs_per_cu; /* 1..8, default 1 */
+  int32_t reduce_targets;   /* chart targets (waves) per reduce workgroup: 1, 2, 4, 8 or 16
+                               (64-1,024 threads), default 4 */
 } puck_launch_config_t;
 int32_t puck_parser_configure(puck_parser_t* p, const puck_launch_config_t* cfg);
 void puck_parser_free(puck_parser_t* p);
@@ -205,6 +214,8 @@ typedef struct {
   int32_t piece_target;   /* see puck_launch_config_t */
   int32_t items_per_lane; /* 1 or 2 */
   int32_t outside_final;  /* Viterbi final layer: outside pass before the decode (insideOutside) */
+  int32_t device_schedule; /* must be 0: see PUCK_FLAG_DEVICE_SCHEDULE (disabled) */
+  int32_t reduce_targets; /* see puck_launch_config_t */
 } puck_batch_config_t;
 typedef struct {
   double device_ms; /* first kernel of the first bucket to the last kernel of the last bucket */

@@ -5,10 +5,14 @@ parent block split into 8 sub-segments per kernel (CoarseParentSymbolGrammarClus
 work groups of 96 work-items (JavaFriendlyGenRuleMultiply.scala:62-63) and batches closed by
 workspace size (WorkSpace.scala:20-41). The knobs of this engine that play those roles are:
 
-  piece_target    how finely each wave's rule stream is cut into work units (1-16 pieces per
-                  stream: the rules one workgroup runs per unit)
+  piece_target    rule-block partition: how finely each wave's rule stream is cut into work units
+                  (1-16 pieces per stream while tiles x chunks x 2 x pieces stays within the
+                  target, so the rules one workgroup runs per unit shrink as the target grows);
+                  the role of the reference's 8 sub-segments per coarse parent block
   items_per_lane  items per lane of the rule kernels: 64 or 128 (cell, split) items per
                   1,024-thread workgroup
+  reduce_targets  workgroup size of the reduce kernel: 1-16 chart targets (waves) per workgroup,
+                  64-1,024 work-items; the role of the reference's fixed 96-item work groups
   persistent      one work-queue launch per span for all rule families, or one launch per family
   bucket_cells    chart cells per length bucket (chart packing)
   streams         buckets in flight on separate HIP streams
@@ -23,12 +27,14 @@ import time
 
 import numpy as np
 
-DEFAULTS = dict(streams=3, bucket_cells=1 << 22, persistent=True, graphs=True, piece_target=512, items_per_lane=2)
+DEFAULTS = dict(streams=3, bucket_cells=1 << 22, persistent=True, graphs=True, piece_target=512, items_per_lane=2,
+                reduce_targets=4)
 SPACE = dict(
     streams=[1, 2, 3, 4],
     bucket_cells=[1 << 20, 1 << 21, 1 << 22, 1 << 23],
-    piece_target=[256, 512, 1024],
+    piece_target=[64, 128, 256, 512, 1024, 2048],
     items_per_lane=[1, 2],
+    reduce_targets=[1, 2, 4, 8, 16],
     persistent=[False, True],
 )
 

@@ -28,6 +28,8 @@ struct BatchConfig {
   int32_t piece_target = 512;
   int32_t items_per_lane = 2;
   bool outside_final = false;     // Viterbi final layer: outside pass before the decode (insideOutside)
+  bool device_schedule = false;   // one persistent launch per pass with device dependency counters
+  int32_t reduce_targets = 4;     // chart targets (waves) per reduce workgroup: 1, 2, 4, 8 or 16
 };
 
 struct BatchStats {

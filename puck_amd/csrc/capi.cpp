@@ -75,7 +75,7 @@ int64_t nodes_needed(const int32_t* offsets, int32_t n) {
 extern "C" {
 
 const char* puck_last_error(void) { return g_last_error.c_str(); }
-int32_t puck_abi_version(void) { return 2; }
+int32_t puck_abi_version(void) { return 3; }
 
 int32_t puck_grammar_load_text(const char* prefix, int32_t semiring, puck_grammar_t** out) {
   return guarded([&]() -> int32_t {
@@ -293,12 +293,14 @@ int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers
       gs.push_back(layers[i]->g);
     }
     const int32_t ipl = (flags & PUCK_FLAG_ONE_ITEM_PER_LANE) ? 1 : 2;
+    if (flags & PUCK_FLAG_DEVICE_SCHEDULE) return set_error(PUCK_ERR_ARG, puck::kDeviceScheduleDisabled);
     auto* h = new puck_parser;
     try {
       h->p.reset(new puck::Parser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, ipl));
       h->p->set_persistent((flags & PUCK_FLAG_PERSISTENT) != 0);
       h->p->use_graphs = (flags & PUCK_FLAG_GRAPHS) != 0;
       h->p->set_outside_final((flags & PUCK_FLAG_OUTSIDE_FINAL) != 0);
+      h->p->set_device_schedule((flags & PUCK_FLAG_DEVICE_SCHEDULE) != 0);
     } catch (...) {
       delete h;
       throw;
@@ -313,9 +315,13 @@ int32_t puck_parser_configure(puck_parser_t* p, const puck_launch_config_t* cfg)
     if (!p || !cfg) return set_error(PUCK_ERR_ARG, "bad argument");
     if (cfg->piece_target < 1 || cfg->queue_wgs_per_cu < 1 || cfg->queue_wgs_per_cu > 8)
       return set_error(PUCK_ERR_ARG, "piece_target must be >= 1 and queue_wgs_per_cu in [1, 8]");
+    const int32_t rt = cfg->reduce_targets;
+    if (rt != 1 && rt != 2 && rt != 4 && rt != 8 && rt != 16)
+      return set_error(PUCK_ERR_ARG, "reduce_targets must be 1, 2, 4, 8 or 16");
     std::lock_guard<std::mutex> lk(p->mu);
     p->p->set_persistent(cfg->persistent != 0);
     p->p->set_launch_shape(cfg->piece_target, cfg->queue_wgs_per_cu);
+    p->p->set_reduce_targets(rt);
     return PUCK_OK;
   });
 }
@@ -550,6 +556,10 @@ int32_t puck_batch_create(const puck_grammar_t* const* layers, int32_t n_layers,
     }
     if (cfg->items_per_lane != 1 && cfg->items_per_lane != 2) return set_error(PUCK_ERR_ARG, "items_per_lane must be 1 or 2");
     if (cfg->piece_target < 1) return set_error(PUCK_ERR_ARG, "piece_target must be >= 1");
+    if (cfg->device_schedule) return set_error(PUCK_ERR_ARG, puck::kDeviceScheduleDisabled);
+    const int32_t rt = cfg->reduce_targets;
+    if (rt != 1 && rt != 2 && rt != 4 && rt != 8 && rt != 16)
+      return set_error(PUCK_ERR_ARG, "reduce_targets must be 1, 2, 4, 8 or 16");
     puck::BatchConfig c;
     c.streams = cfg->streams;
     c.bucket_cells = cfg->bucket_cells;
@@ -558,6 +568,8 @@ int32_t puck_batch_create(const puck_grammar_t* const* layers, int32_t n_layers,
     c.piece_target = cfg->piece_target;
     c.items_per_lane = cfg->items_per_lane;
     c.outside_final = cfg->outside_final != 0;
+    c.device_schedule = cfg->device_schedule != 0;
+    c.reduce_targets = rt;
     auto* h = new puck_batch;
     try {
       h->b.reset(new puck::BatchParser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, c));

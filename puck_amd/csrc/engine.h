@@ -10,12 +10,19 @@
 #include <algorithm>
 #include <cstdint>
 #include <map>
+#include <stdexcept>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "grammar.h"
 #include "kernels.h"
+
+namespace puck {
+constexpr const char* kDeviceScheduleDisabled =
+    "device-scheduled passes are disabled: their one launch per pass did not finish on the GPU (cause not "
+    "found); parse with per-span launches";
+}
 
 namespace puck {
 
@@ -225,6 +232,20 @@ class Parser {
     drop_graphs();
   }
   bool outside_final = false;
+  // Device-scheduled passes: every pass whose (group, span) stages each fit in one slice of half the
+  // partial buffer runs as one persistent k_pass launch that pulls the rule and reduce units of all
+  // its spans from one device queue, with per-block dependency counters (see PassStage); other
+  // passes keep one launch per span.
+  // Disabled: on the GPU the first device-scheduled parse did not return within 150 s (twice, the
+  // second time with every cross-unit access agent-coherent and no L2-wide fences), and no cause has
+  // been shown; enabling it is rejected until one is.
+  void set_device_schedule(bool on) {
+    if (on) throw std::invalid_argument(kDeviceScheduleDisabled);
+    if (device_schedule == on) return;
+    device_schedule = on;
+    drop_graphs();
+  }
+  bool device_schedule = false;
   // Size of the rule kernels' partial buffer (default: 30% of mem_bytes, at least 64 MiB); a
   // smaller buffer cuts a span's cells into more slices, never changing results.
   void set_partial_bytes(int64_t bytes) {
@@ -233,8 +254,18 @@ class Parser {
   }
   // Drops every captured graph (their launches no longer match the configuration).
   void drop_graphs();
+  // Workgroup size of the reduce launches: targets (waves) per workgroup, one of 1, 2, 4, 8, 16 (64
+  // to 1,024 threads); 4 measured fastest on the exhaustive and coarse-to-fine workloads. The
+  // device-scheduled pass always reduces 16 targets per unit.
+  void set_reduce_targets(int32_t targets) {
+    if (targets != 1 && targets != 2 && targets != 4 && targets != 8 && targets != kReduceTargets)
+      throw std::invalid_argument("reduce targets per workgroup must be 1, 2, 4, 8 or 16");
+    reduce_targets_ = targets;
+    drop_graphs();
+  }
   int32_t piece_target_ = 512;
   int32_t queue_wgs_per_cu_ = 1;
+  int32_t reduce_targets_ = 4;
   // Graph mode of parse(): the device work of a batch is captured into a HIP graph the second
   // time a batch of the same sentence lengths (in order) is seen, and replayed after that.
   bool use_graphs = false;
@@ -278,7 +309,7 @@ class Parser {
   // Workspace (grown on demand).
   struct Buf { void* p = nullptr; size_t bytes = 0; };
   Buf inside_, outside_, in_scale_, out_scale_, masks_, partial_, plan_, tags_, trees_, mbr_,
-      pos_scale_, aux_, mplan_, apps_, queue_, jobs_;
+      pos_scale_, aux_, mplan_, apps_, queue_, jobs_, pass_, pcnt_;
   int32_t queue_next_ = 0;
   // No-alloc mode: ensure() carves buffers from [base, base + size) instead of allocating.
   struct Arena { char* base = nullptr; size_t size = 0, used = 0; };
@@ -334,7 +365,23 @@ class Parser {
     std::vector<int32_t> apps_slot;
     int32_t* d_perm = nullptr;  // visiting order of the reduce: the group's largest family's order
   };
-  std::vector<std::pair<int32_t, int32_t>> group_slices(Batch& b, int32_t li, int32_t rid, int32_t span);
+  // One slice of a (reduce group, span): its families' rule-kernel arguments and its reduce.
+  struct GroupSlice {
+    int32_t nfam = 0;
+    int32_t fid[kMaxGroupFamilies] = {};
+    bool valid[kMaxGroupFamilies] = {};  // the family has items in the slice
+    FamArgs fa[kMaxGroupFamilies] = {};
+    dim3 grid[kMaxGroupFamilies];        // plain launch grid: tiles, chunks, pieces
+    size_t lds[kMaxGroupFamilies] = {};
+    float* partial[kMaxGroupFamilies] = {};
+    int32_t apps_slot[kMaxGroupFamilies] = {};
+    RedArgs ra = {};
+    dim3 rgrid;
+  };
+  size_t group_slice_args(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks, const PlannedSlice* ps,
+                          int32_t k0, int32_t kc, float* base, GroupSlice* out);
+  std::vector<std::pair<int32_t, int32_t>> group_slices(Batch& b, int32_t li, int32_t rid, int32_t span,
+                                                        int64_t budget_floats = -1);
   int64_t family_items(Batch& b, int32_t li, int32_t fid, int32_t span, int32_t k0, int32_t kc);
   // Passes of a call in execution order: (layer, outside pass). `what` as in run_batch.
   std::vector<std::pair<int32_t, bool>> pass_schedule(int32_t what, int32_t chart_layer) const;
@@ -346,6 +393,15 @@ class Parser {
   // The four plan launches over every job of layer li's masked passes, on the parser's stream.
   void launch_plans(Batch& b, int32_t li);
   void run_groups(Batch& b, int32_t li, bool outside, bool use_masks);
+  // Device-scheduled passes of a call: laid out with the batch (prepare_dev_passes), launched in
+  // schedule order in place of run_groups (run_dev_pass returns false for a pass that keeps its
+  // per-span launches).
+  void prepare_dev_passes(Batch& b, const std::vector<std::pair<int32_t, bool>>& schedule);
+  bool build_dev_pass(Batch& b, int32_t li, bool outside, bool use_masks, const void* pass_plan,
+                      std::vector<PassStage>* stages, std::vector<PassTile>* tiles, std::vector<int32_t>* expected,
+                      int32_t* counters, void* dev_pass);
+  bool run_dev_pass(Batch& b, int32_t li, bool outside, bool use_masks);
+  bool dev_pass_ran_ = false;  // a k_pass ran since the last error check
   void run_group(Batch& b, int32_t li, int32_t rid, int32_t span, bool use_masks,
                  const std::vector<PlannedSlice>* planned);
   void timed_begin();

@@ -132,6 +132,22 @@ __device__ inline float semi_add(float acc, float v) {
   else return acc + v;
 }
 
+// Loads and stores of data that another workgroup of the same launch produces or consumes (the
+// device-scheduled pass, COH = true): relaxed agent-scope atomics, which the compiler emits as
+// global loads and stores with the sc1 bit, coherent across the XCDs' L2 caches without a
+// whole-cache writeback or invalidate. Other launches use plain accesses (COH = false).
+typedef __attribute__((address_space(1))) float GlobalF32;
+template <bool COH>
+__device__ __forceinline__ float ld_f(const float* p) {
+  if constexpr (COH) return __hip_atomic_load((GlobalF32*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void st_f(float* p, float v) {
+  if constexpr (COH) __hip_atomic_store((GlobalF32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
 // ------------------------------------------------------------------------------------------------
 // s_waitcnt immediate (gfx9 encoding) that waits for lgkmcnt 0 only: vmcnt 63, expcnt 7.
 constexpr int kWaitLgkm0 = 0xC07F;
@@ -154,8 +170,9 @@ struct RulesLds {
 
 // One work unit of a rule family: tile `tile` of ITEMS items x operand chunk `chunk` x piece
 // `piece` of every wave's rule stream. Every thread of the workgroup calls it with the same
-// arguments; it contains workgroup barriers before the rule loop only.
-template <int IPL, int SEMI, bool UNARY, bool SKIP>
+// arguments; it contains workgroup barriers before the rule loop only. COH: operand rows are read
+// and partial rows written with agent-coherent accesses (see ld_f).
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool COH = false>
 __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk, int piece, float* smem,
                                            RulesLds<64 * IPL>& sh) {
   constexpr int ITEMS = 64 * IPL;
@@ -308,9 +325,9 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       const uint32_t b0 = min(row0, (uint32_t)rows - 1u) * 4u, b1 = min(row1, (uint32_t)rows - 1u) * 4u;
       float v0[IPW], v1[IPW];
 #pragma unroll
-      for (int k = 0; k < IPW; ++k) v0[k] = *reinterpret_cast<const float*>(base[k] + b0);
+      for (int k = 0; k < IPW; ++k) v0[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b0));
 #pragma unroll
-      for (int k = 0; k < IPW; ++k) v1[k] = *reinterpret_cast<const float*>(base[k] + b1);
+      for (int k = 0; k < IPW; ++k) v1[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b1));
       if (in0) {
         float* d = dst + row0 * a.row_stride + sw;
 #pragma unroll
@@ -460,14 +477,14 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     if constexpr (IPL == 2 && !SKIP) {
       if (__builtin_amdgcn_readfirstlane(pair_v)) {
         const int m = (tile0 >> 1) + lane;
-        if (2 * m < num_items) out[m] = semi_pair<SEMI>(v[0], v[1]);
+        if (2 * m < num_items) st_f<COH>(out + m, semi_pair<SEMI>(v[0], v[1]));
         return;
       }
     }
 #pragma unroll
     for (int i = 0; i < IPL; ++i) {
       const int item = tile0 + lane * IPL + i;
-      if (item < num_items) out[item] = v[i];
+      if (item < num_items) st_f<COH>(out + item, v[i]);
     }
   };
   auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, float (*da)[IPL], float (*db)[IPL], int h) {
@@ -783,7 +800,7 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanBatchArgs p) {
 }
 
 // Combine partials: target[cell][t] = (+) over (family, slot) of t over the cell's items.
-// One workgroup = 64 consecutive cells x kReduceTargets targets, one wave per target. For each
+// One workgroup = 64 consecutive cells x TGT targets, one wave per target (TGT = 1..16). For each
 // partial row feeding its target, a wave streams the items of its 64 cells (one contiguous range)
 // through a per-wave LDS buffer with coalesced loads; each lane then folds its own cell's items.
 // The block's results are transposed through LDS so that each cell's targets are written as one
@@ -791,16 +808,21 @@ __global__ __launch_bounds__(1024) void k_plan_scan(PlanBatchArgs p) {
 // Partial values a reduce wave keeps in flight within one family's run of a target's entries.
 constexpr int kReduceBatch = 8;
 
-template <int SEMI>
-__global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
-  constexpr int CAP = 512;  // floats per wave buffer
-  __shared__ float s_buf[kReduceTargets][CAP];
-  __shared__ float s_out[kReduceTargets][65];
+constexpr int kReduceCap = 512;  // floats per wave buffer of the reduce
+
+// One reduce unit: cells [64 bx, 64 bx + 64) of the slice x targets [TGT by, TGT by + TGT), by a
+// workgroup of TGT waves (k_reduce's block, or with TGT = kReduceTargets a unit of the
+// device-scheduled pass). LDS: s_buf (TGT x kReduceCap floats) and s_out (TGT x 65). COH: partial
+// rows are read and the chart written with agent-coherent accesses (see ld_f).
+template <int SEMI, int TGT, bool COH = false>
+__device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, float (*s_buf)[kReduceCap],
+                                            float (*s_out)[65]) {
+  constexpr int CAP = kReduceCap;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int k0 = blockIdx.x * 64;
+  const int k0 = bx * 64;
   const int kend = min(k0 + 64, a.num_cells);
-  const int t = blockIdx.y * kReduceTargets + w;
+  const int t = by * TGT + w;
   const int my_k = k0 + lane;  // position in the visiting order
   const int my_c = (a.perm != nullptr && my_k < kend) ? a.perm[my_k] : my_k;  // slice cell
   float acc = semi_zero<SEMI>();
@@ -836,8 +858,10 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
             // Two items at a time: eight loads in flight, folded in (item, slot) order.
             int j = lo;
             for (; j + 2 <= hi; j += 2) {
-              const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
-              const float w0 = p0[j + 1], w1 = p1[j + 1], w2 = p2[j + 1], w3 = p3[j + 1];
+              const float v0 = ld_f<COH>(p0 + j), v1 = ld_f<COH>(p1 + j), v2 = ld_f<COH>(p2 + j),
+                          v3 = ld_f<COH>(p3 + j);
+              const float w0 = ld_f<COH>(p0 + j + 1), w1 = ld_f<COH>(p1 + j + 1), w2 = ld_f<COH>(p2 + j + 1),
+                          w3 = ld_f<COH>(p3 + j + 1);
               if (SEMI == kViterbi) {
                 acc = fmaxf(fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3))), fmaxf(fmaxf(w0, w1), fmaxf(w2, w3)));
               } else {
@@ -852,7 +876,8 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
               }
             }
             if (j < hi) {
-              const float v0 = p0[j], v1 = p1[j], v2 = p2[j], v3 = p3[j];
+              const float v0 = ld_f<COH>(p0 + j), v1 = ld_f<COH>(p1 + j), v2 = ld_f<COH>(p2 + j),
+                          v3 = ld_f<COH>(p3 + j);
               if (SEMI == kViterbi) {
                 acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
               } else {
@@ -865,7 +890,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
           }
           for (; q < kr; ++q) {
             const float* p0 = base + (size_t)__builtin_amdgcn_readlane(ls, q) * pitch;
-            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, p0[j]);
+            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, ld_f<COH>(p0 + j));
           }
         }
         k = kr;
@@ -912,7 +937,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
               v[u] = semi_zero<SEMI>();
               if (m + u < total) {
                 const int sl = __builtin_amdgcn_readlane(ls, ent);
-                if (live) v[u] = part[(size_t)sl * pitch + ((size_t)blockIdx.x * uc + jj) * 64 + lane];
+                if (live) v[u] = ld_f<COH>(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane);
                 if (++jj == uc) {
                   jj = 0;
                   ++ent;
@@ -947,7 +972,7 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
 #pragma unroll
             for (int u = 0; u < CAP / 64; ++u) {
               const int i = lane + 64 * u;
-              if (i < len) buf[i] = p[pb + i];
+              if (i < len) buf[i] = ld_f<COH>(p + pb + i);
             }
             // LDS operations of one wave complete in order; the fences keep the compiler from
             // moving the reads above the writes (or the next piece's writes above these reads).
@@ -966,10 +991,10 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   }
   s_out[w][lane] = acc;
   __syncthreads();
-  // Write: thread i -> cell i / kReduceTargets, target i % kReduceTargets.
-  const int tt = threadIdx.x % kReduceTargets;
-  const int kk = threadIdx.x / kReduceTargets;
-  const int t_out = blockIdx.y * kReduceTargets + tt;
+  // Write: thread i -> cell i / TGT, target i % TGT.
+  const int tt = threadIdx.x % TGT;
+  const int kk = threadIdx.x / TGT;
+  const int t_out = by * TGT + tt;
   const int k = k0 + kk;
   if (k >= a.num_cells || t_out >= a.n_t) return;
   float v = s_out[tt][kk];
@@ -982,7 +1007,112 @@ __global__ __launch_bounds__(64 * kReduceTargets) void k_reduce(RedArgs a) {
   }
   if (a.act[0] != nullptr && v == semi_zero<SEMI>() && !(a.root_init && b == 0 && e == n)) return;
   const int c = cell_abs(a.cell_offsets, sent, n, a.t_half, b, e);
-  a.chT[(size_t)c * a.cell_size + t_out] = v;
+  st_f<COH>(a.chT + (size_t)c * a.cell_size + t_out, v);
+}
+
+template <int SEMI, int TGT>
+__global__ __launch_bounds__(64 * TGT) void k_reduce(RedArgs a) {
+  __shared__ float s_buf[TGT][kReduceCap];
+  __shared__ float s_out[TGT][65];
+  reduce_unit<SEMI, TGT>(a, blockIdx.x, blockIdx.y, s_buf, s_out);
+}
+
+// Device-scheduled pass (see PassStage). Waits are taken by thread 0 on device-scope counters. The
+// data that crosses units (chart cells, partial rows) is written and read with agent-coherent
+// accesses (rules_unit and reduce_unit with COH), so no unit needs an agent-scope fence, which on
+// this GPU writes back or invalidates a whole L2 cache. A unit publishes by waiting in every wave
+// until its stores are acknowledged (unit_done), then raising its counters from thread 0; a waiting
+// unit's loads come after its wait and the workgroup barrier.
+__device__ inline void pass_wait(int32_t* c, int32_t target, int32_t* err) {
+  for (int it = 0;; ++it) {
+    if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    if (it >= kPassSpinLimit) {
+      __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(8);
+  }
+}
+
+// Every wave: wait until this unit's memory operations have completed (stores acknowledged at the
+// coherence point), then the workgroup barrier; only then does thread 0 raise the counters.
+__device__ __forceinline__ void unit_done() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+}
+
+template <int IPL, int SEMI, bool SKIP>
+__global__ __launch_bounds__(kBlockThreads) void k_pass(PassArgs p) {
+  constexpr int ITEMS = 64 * IPL;
+  extern __shared__ __align__(16) float smem[];
+  __shared__ RulesLds<ITEMS> sh;
+  __shared__ int32_t s_unit;
+  // The reduce's buffers live in the dynamic LDS the rule units stage operands in.
+  float (*s_buf)[kReduceCap] = reinterpret_cast<float (*)[kReduceCap]>(smem);
+  float (*s_out)[65] = reinterpret_cast<float (*)[65]>(smem + kReduceTargets * kReduceCap);
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = atomicAdd(p.queue, 1);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);
+    __syncthreads();  // every thread has read s_unit before thread 0 writes the next one
+    if (u >= p.total_units) break;
+    int lo = 0, hi = p.num_stages;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (p.stages[mid].first_unit <= u) lo = mid; else hi = mid;
+    }
+    const PassStage& st = p.stages[lo];
+    int32_t* cnt = p.counters + st.cnt;
+    int r = u - st.first_unit;
+    if (st.kind == 0) {
+      int f = 0;
+      while (f + 1 < st.nfam && st.fam_units[f + 1] <= r) ++f;
+      r -= st.fam_units[f];
+      const FamArgs& fa = st.fam[f];
+      const int tiles = st.fam_tiles[f];
+      const int tile = r % tiles;
+      const int cp = r / tiles;
+      const int chunk = cp % fa.num_chunks;
+      const int piece = cp / fa.num_chunks;
+      const PassTile tt = p.tiles[st.tile_base[f] + tile];
+      if (threadIdx.x == 0) {
+        if (st.war_cnt >= 0) pass_wait(p.counters + st.war_cnt + 2 * st.war_nb + 1, st.war_units, p.error);
+        if (st.dep_cnt >= 0) {
+          int32_t* dc = p.counters + st.dep_cnt + st.dep_nb;
+          if (tt.dep_lo == -2) {
+            pass_wait(p.counters + st.dep_cnt + 2 * st.dep_nb + 1, st.dep_units, p.error);
+          } else {
+            for (int b = tt.dep_lo; b <= tt.dep_hi; ++b) pass_wait(dc + b, st.dep_gy, p.error);
+          }
+        }
+      }
+      __syncthreads();
+      if (fa.kind == 7) rules_unit<IPL, SEMI, true, false, true>(fa, tile, chunk, piece, smem, sh);
+      else if (SKIP && fa.skip_layout) rules_unit<IPL, SEMI, false, SKIP, true>(fa, tile, chunk, piece, smem, sh);
+      else rules_unit<IPL, SEMI, false, false, true>(fa, tile, chunk, piece, smem, sh);
+      unit_done();
+      if (threadIdx.x == 0) {
+        if (tt.out_lo >= 0)
+          for (int b = tt.out_lo; b <= tt.out_hi; ++b) atomicAdd(cnt + b, 1);
+        atomicAdd(cnt + 2 * st.nb, 1);
+      }
+    } else {
+      const int bx = r / st.gy, by = r - (r / st.gy) * st.gy;
+      if (threadIdx.x == 0) {
+        if (st.exp_off >= 0) pass_wait(cnt + bx, p.expected[st.exp_off + bx], p.error);
+        else pass_wait(cnt + 2 * st.nb, st.rule_units, p.error);
+      }
+      __syncthreads();
+      reduce_unit<SEMI, kReduceTargets, true>(st.red, bx, by, s_buf, s_out);
+      unit_done();
+      if (threadIdx.x == 0) {
+        atomicAdd(cnt + st.nb + bx, 1);
+        atomicAdd(cnt + 2 * st.nb + 1, 1);
+      }
+    }
+  }
 }
 
 __global__ void k_fill(float* p, size_t n, float v) {
@@ -1766,6 +1896,24 @@ struct Parser::Batch {
   std::vector<LayerJobs> layer_jobs;
   std::vector<PlanJob> h_jobs;
   PlanJob* d_jobs = nullptr;
+  // Device-scheduled passes, one entry per pass of the call in schedule order (on = false: the
+  // pass keeps its per-span launches). Stages, tiles and expected counts of all passes travel in
+  // one table (pass_), counters in pcnt_ ([0] is the error flag).
+  struct DevPass {
+    int32_t li = 0;
+    bool outside = false;
+    bool on = false;
+    bool skip = false;          // some family streams block-major
+    int32_t stage0 = 0, num_stages = 0, total_units = 0;
+    int32_t cnt_off = 0, num_counters = 0;  // queue head at cnt_off, group counters after it
+    double apps = 0;            // rule applications of unmasked families
+    int32_t apps_first = -1, apps_count = 0;  // plan counters of masked families
+  };
+  std::vector<DevPass> dev_passes;
+  size_t next_dev = 0;
+  size_t tiles_byte_off = 0, exp_byte_off = 0;
+  std::vector<char> h_pass_blob;
+  int32_t num_pass_counters = 0;
   int32_t mask_in = 0;   // which half of masks_ holds the current input masks
   bool has_masks = false;
   bool has_scales = false;
@@ -1847,6 +1995,13 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
                          (const void*)k_rules_q<2, kLogSum, false, true>, (const void*)k_rules_q<1, kLogSum, false, false>,
                          (const void*)k_rules_q<1, kLogSum, true, false>, (const void*)k_rules_q<1, kLogSum, false, true>})
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  for (const void* fn : {(const void*)k_pass<2, kViterbi, false>, (const void*)k_pass<2, kViterbi, true>,
+                         (const void*)k_pass<2, kReal, false>, (const void*)k_pass<2, kReal, true>,
+                         (const void*)k_pass<2, kLogSum, false>, (const void*)k_pass<2, kLogSum, true>,
+                         (const void*)k_pass<1, kViterbi, false>, (const void*)k_pass<1, kViterbi, true>,
+                         (const void*)k_pass<1, kReal, false>, (const void*)k_pass<1, kReal, true>,
+                         (const void*)k_pass<1, kLogSum, false>, (const void*)k_pass<1, kLogSum, true>})
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   int cus = 0;
   HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
   num_cus_ = std::max(1, cus);
@@ -1854,7 +2009,7 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
 
 Parser::~Parser() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_,
-                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_})
+                 &trees_, &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_, &pass_, &pcnt_})
     if (b->p) (void)hipFree(b->p);
   drop_graphs();
   if (ev0_) (void)hipEventDestroy(ev0_);
@@ -1909,7 +2064,7 @@ void Parser::ensure(Buf& b, size_t bytes) {
 int64_t Parser::workspace_bytes() const {
   int64_t total = 0;
   for (const Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_, &trees_,
-                       &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_})
+                       &mbr_, &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_, &pass_, &pcnt_})
     total += (int64_t)b->bytes;
   return total;
 }
@@ -2064,6 +2219,7 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.t_half = F.t_half;
   a.row_stride = 64 * L.ipl + (L.ipl == 1 ? 1 : 2);
   a.num_chunks = F.num_chunks;
+  a.skip_layout = F.skip_layout ? 1 : 0;
   // Few tiles: cut the rule streams into pieces so that the launch still fills the chip (about
   // two workgroups per CU); large launches run whole streams.
   const int64_t tiles = (nitems + items_tile - 1) / items_tile;
@@ -2208,6 +2364,12 @@ void Parser::drop_graphs() {
 
 void Parser::flush_timing() {
   HIP_CHECK(hipStreamSynchronize(stream_));
+  if (dev_pass_ran_) {
+    dev_pass_ran_ = false;
+    int32_t err = 0;
+    HIP_CHECK(hipMemcpy(&err, pcnt_.p, 4, hipMemcpyDeviceToHost));
+    if (err != 0) throw std::runtime_error("device-scheduled pass: a dependency wait timed out");
+  }
   std::vector<double> plan_apps(apps_next_, 0.0);
   if (apps_next_ > 0 && !pending_.empty()) {
     HIP_CHECK(hipMemcpyAsync(plan_apps.data(), apps_.p, (size_t)apps_next_ * 8, hipMemcpyDeviceToHost, stream_));
@@ -2247,12 +2409,13 @@ void Parser::flush_timing() {
 
 // Slices of a group's cells at one span whose partials fit the partial buffer (num_slots x pitch
 // floats per family, pitch = items rounded up to 256).
-std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t li, int32_t rid, int32_t span) {
+std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t li, int32_t rid, int32_t span,
+                                                              int64_t budget_floats) {
   Layer& L = *layers_[li];
   const DevReduce& R = L.red[rid];
   const int32_t ncells = B.span_count[span];
   std::vector<std::pair<int32_t, int32_t>> out;
-  const int64_t budget_floats = (int64_t)(partial_.bytes / 4);
+  if (budget_floats < 0) budget_floats = (int64_t)(partial_.bytes / 4);
   int64_t fixed = 0;
   // Row padding: pitches round up to 256 values, block-major uniform rows to whole 64-cell blocks.
   for (int fid : R.families) {
@@ -2419,6 +2582,256 @@ void Parser::prepare_plans(Batch& B, const std::vector<std::pair<int32_t, bool>>
   }
 }
 
+// Lays out the device-scheduled passes of a call (see PassStage) and their table.
+void Parser::prepare_dev_passes(Batch& B, const std::vector<std::pair<int32_t, bool>>& schedule) {
+  B.dev_passes.clear();
+  B.next_dev = 0;
+  B.h_pass_blob.clear();
+  std::vector<PassStage> stages;
+  std::vector<PassTile> tiles;
+  std::vector<int32_t> expected;
+  int32_t counters = 64;  // [0, 64): the error flag, on a cache line of its own
+  size_t masked_pass = 0;
+  for (auto [li, outside] : schedule) {
+    Batch::DevPass dp;
+    dp.li = li;
+    dp.outside = outside;
+    const bool use_masks = li > 0;
+    const Batch::PassPlan* pp = use_masks ? &B.pass_plans[masked_pass++] : nullptr;
+    if (device_schedule) {
+      const size_t s0 = stages.size(), t0 = tiles.size(), e0 = expected.size();
+      const int32_t c0 = counters;
+      if (!build_dev_pass(B, li, outside, use_masks, pp, &stages, &tiles, &expected, &counters, &dp)) {
+        stages.resize(s0);
+        tiles.resize(t0);
+        expected.resize(e0);
+        counters = c0;
+        dp = Batch::DevPass{};
+        dp.li = li;
+        dp.outside = outside;
+      }
+    }
+    B.dev_passes.push_back(dp);
+  }
+  B.num_pass_counters = counters;
+  if (stages.empty()) return;
+  auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+  B.tiles_byte_off = align(stages.size() * sizeof(PassStage));
+  B.exp_byte_off = align(B.tiles_byte_off + tiles.size() * sizeof(PassTile));
+  B.h_pass_blob.assign(B.exp_byte_off + std::max<size_t>(expected.size(), 1) * 4, 0);
+  std::memcpy(B.h_pass_blob.data(), stages.data(), stages.size() * sizeof(PassStage));
+  if (!tiles.empty()) std::memcpy(B.h_pass_blob.data() + B.tiles_byte_off, tiles.data(), tiles.size() * sizeof(PassTile));
+  if (!expected.empty()) std::memcpy(B.h_pass_blob.data() + B.exp_byte_off, expected.data(), expected.size() * 4);
+}
+
+// One pass as stages; false (nothing appended) when some (group, span) needs more than one slice of
+// half the partial buffer or has more families than a stage holds.
+bool Parser::build_dev_pass(Batch& B, int32_t li, bool outside, bool use_masks, const void* pass_plan,
+                            std::vector<PassStage>* stages, std::vector<PassTile>* tiles,
+                            std::vector<int32_t>* expected, int32_t* counters, void* dev_pass) {
+  const Batch::PassPlan* pp = static_cast<const Batch::PassPlan*>(pass_plan);
+  Batch::DevPass& dp = *static_cast<Batch::DevPass*>(dev_pass);
+  Layer& L = *layers_[li];
+  const auto groups = outside ? outside_groups(B) : inside_groups(B);
+  const int64_t half = (int64_t)(partial_.bytes / 4 / 2);
+  for (auto [rid, span] : groups) {
+    if (B.span_count[span] == 0) continue;
+    if (L.red[rid].families.size() > (size_t)kMaxPassFamilies) return false;
+    try {
+      if (group_slices(B, li, rid, span, half).size() != 1) return false;
+    } catch (const std::runtime_error&) {
+      return false;
+    }
+  }
+  // first_cell[w][i]: cells of span w that belong to sentences before i (cells are sentence-major).
+  std::map<int32_t, std::vector<int32_t>> first_cell;
+  auto firsts = [&](int32_t w) -> const std::vector<int32_t>& {
+    auto it = first_cell.find(w);
+    if (it != first_cell.end()) return it->second;
+    std::vector<int32_t> f(B.n + 1, 0);
+    for (int32_t i = 0; i < B.n; ++i) f[i + 1] = f[i] + std::max(0, B.lengths[i] - w + 1);
+    return first_cell.emplace(w, std::move(f)).first->second;
+  };
+  struct Prev { int32_t cnt, nb, gy, dunits, span; };
+  std::vector<Prev> prev;
+  dp.on = true;
+  dp.stage0 = (int32_t)stages->size();
+  dp.cnt_off = *counters;
+  int32_t cnt = *counters + 1;  // after the queue head
+  int32_t unit = 0;
+  int32_t ei = 0;  // index among the pass's groups with cells
+  for (size_t gi = 0; gi < groups.size(); ++gi) {
+    const auto [rid, span] = groups[gi];
+    const int32_t kc = B.span_count[span];
+    if (kc == 0) continue;
+    const DevReduce& R = L.red[rid];
+    const PlannedSlice* ps = use_masks ? &pp->groups[gi][0] : nullptr;
+    GroupSlice gs;
+    float* base = (float*)partial_.p + (size_t)(ei % 2) * (size_t)half;
+    group_slice_args(B, li, rid, span, use_masks, ps, 0, kc, base, &gs);
+    const int32_t nb = (kc + 63) / 64;
+    const int32_t gy = (R.n_t + kReduceTargets - 1) / kReduceTargets;
+    PassStage rs{};
+    rs.kind = 0;
+    rs.first_unit = unit;
+    rs.cnt = cnt;
+    rs.nb = nb;
+    rs.gy = gy;
+    rs.dep_cnt = rs.war_cnt = -1;
+    if (ei >= 1) {
+      const Prev& p = prev[ei - 1];
+      rs.dep_cnt = p.cnt;
+      rs.dep_nb = p.nb;
+      rs.dep_gy = p.gy;
+      rs.dep_units = p.dunits;
+    }
+    if (ei >= 2) {
+      const Prev& p = prev[ei - 2];
+      rs.war_cnt = p.cnt;
+      rs.war_nb = p.nb;
+      rs.war_units = p.dunits;
+    }
+    std::vector<int32_t> exp_blocks(nb, 0);
+    int32_t ru = 0;
+    for (int32_t fi = 0; fi < gs.nfam; ++fi) {
+      if (!gs.valid[fi]) continue;
+      const FamArgs& a = gs.fa[fi];
+      const int32_t k = rs.nfam++;
+      const int32_t T = (int32_t)gs.grid[fi].x, C = (int32_t)gs.grid[fi].y, P = (int32_t)gs.grid[fi].z;
+      rs.fam[k] = a;
+      rs.fam_units[k] = ru;
+      rs.fam_tiles[k] = T;
+      rs.tile_base[k] = (int32_t)tiles->size();
+      ru += T * C * P;
+      dp.skip |= a.skip_layout != 0;
+      if (use_masks) {
+        // Planned items follow the family's own cell order: tiles touch any block.
+        for (int32_t t = 0; t < T; ++t) tiles->push_back(PassTile{-1, -1, ei >= 1 ? -2 : 1, ei >= 1 ? -2 : 0});
+        const int32_t slot = gs.apps_slot[fi];
+        if (slot >= 0) {
+          if (dp.apps_first < 0) dp.apps_first = slot;
+          dp.apps_count = std::max(dp.apps_count, slot - dp.apps_first + 1);
+        }
+        continue;
+      }
+      dp.apps += (double)family_items(B, li, gs.fid[fi], span, 0, kc) * L.fam[gs.fid[fi]].rule_apps_per_item;
+      const int32_t ti = 64 * L.ipl;
+      for (int32_t t = 0; t < T; ++t) {
+        int32_t b_lo, b_hi, c_lo, c_hi;
+        if (a.prefix == nullptr) {
+          const int32_t uc = a.uniform_count;
+          if (a.pair_splits) {
+            b_lo = b_hi = t / uc;  // 64 pairs: one row of one block
+          } else {
+            b_lo = (t * L.ipl) / uc;
+            b_hi = std::min((t * L.ipl + L.ipl - 1) / uc, nb - 1);
+          }
+          c_lo = b_lo * 64;
+          c_hi = std::min(b_hi * 64 + 63, kc - 1);
+        } else {
+          const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + gs.fid[fi]];
+          const int32_t* pre = B.h_prefix.data() + po;
+          const int32_t n_items = pre[kc] - pre[0];
+          const int32_t first = t * ti, last = std::min(first + ti, n_items) - 1;
+          c_lo = (int32_t)(std::upper_bound(pre, pre + kc + 1, pre[0] + first) - pre) - 1;
+          c_hi = (int32_t)(std::upper_bound(pre, pre + kc + 1, pre[0] + last) - pre) - 1;
+          b_lo = c_lo / 64;
+          b_hi = c_hi / 64;
+        }
+        PassTile pt{b_lo, b_hi, 1, 0};
+        if (ei >= 1) {
+          const int32_t i_lo = B.h_cells[B.span_start[span] + c_lo] >> 8;
+          const int32_t i_hi = B.h_cells[B.span_start[span] + c_hi] >> 8;
+          const std::vector<int32_t>& f = firsts(prev[ei - 1].span);
+          const int32_t c0 = f[i_lo], c1 = f[i_hi + 1];
+          if (c0 < c1) {
+            pt.dep_lo = c0 / 64;
+            pt.dep_hi = (c1 - 1) / 64;
+          }
+        }
+        tiles->push_back(pt);
+        for (int32_t b = b_lo; b <= b_hi; ++b) exp_blocks[b] += C * P;
+      }
+    }
+    rs.num_units = ru;
+    rs.rule_units = ru;
+    unit += ru;
+    PassStage ds = rs;
+    ds.kind = 1;
+    ds.first_unit = unit;
+    ds.num_units = nb * gy;
+    ds.nfam = 0;
+    ds.red = gs.ra;
+    ds.exp_off = -1;
+    if (!use_masks) {
+      ds.exp_off = (int32_t)expected->size();
+      expected->insert(expected->end(), exp_blocks.begin(), exp_blocks.end());
+    }
+    unit += nb * gy;
+    if (ru > 0) stages->push_back(rs);
+    stages->push_back(ds);
+    prev.push_back(Prev{cnt, nb, gy, nb * gy, span});
+    cnt += 2 * nb + 2;
+    ++ei;
+  }
+  dp.num_stages = (int32_t)stages->size() - dp.stage0;
+  dp.total_units = unit;
+  dp.num_counters = cnt - dp.cnt_off;
+  *counters = cnt;
+  return true;
+}
+
+// Runs the next pass of the call as one k_pass launch if it was laid out for it.
+bool Parser::run_dev_pass(Batch& B, int32_t li, bool outside, bool use_masks) {
+  if (B.next_dev >= B.dev_passes.size()) return false;
+  const Batch::DevPass& dp = B.dev_passes[B.next_dev++];
+  if (dp.li != li || dp.outside != outside) throw std::logic_error("pass out of the laid-out schedule");
+  if (!dp.on) return false;
+  if (use_masks) {
+    if (B.next_pass >= B.pass_plans.size()) throw std::logic_error("masked pass without its plan");
+    ++B.next_pass;
+  }
+  const Layer& L = *layers_[li];
+  PassArgs pa{};
+  char* blob = (char*)pass_.p;
+  pa.stages = (const PassStage*)blob + dp.stage0;
+  pa.tiles = (const PassTile*)(blob + B.tiles_byte_off);
+  pa.expected = (const int32_t*)(blob + B.exp_byte_off);
+  pa.counters = (int32_t*)pcnt_.p;
+  pa.queue = pa.counters + dp.cnt_off;
+  pa.error = pa.counters;
+  pa.num_stages = dp.num_stages;
+  pa.total_units = dp.total_units;
+  hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, stream_, (float*)(pa.counters + dp.cnt_off),
+                     (size_t)dp.num_counters, 0.0f);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (timing) {
+    t0 = next_event();
+    t1 = next_event();
+    HIP_CHECK(hipEventRecord(t0, stream_));
+  }
+  const dim3 grid((unsigned)std::max(1, num_cus_));
+  launch_semi(L.g.semiring, [&](auto semi) {
+    constexpr int S = decltype(semi)::value;
+    if (L.ipl == 2) {
+      if (dp.skip) hipLaunchKernelGGL((k_pass<2, S, true>), grid, dim3(kBlockThreads), kLdsBudgetBytes, stream_, pa);
+      else hipLaunchKernelGGL((k_pass<2, S, false>), grid, dim3(kBlockThreads), kLdsBudgetBytes, stream_, pa);
+    } else {
+      if (dp.skip) hipLaunchKernelGGL((k_pass<1, S, true>), grid, dim3(kBlockThreads), kLdsBudgetBytes, stream_, pa);
+      else hipLaunchKernelGGL((k_pass<1, S, false>), grid, dim3(kBlockThreads), kLdsBudgetBytes, stream_, pa);
+    }
+  });
+  HIP_CHECK(hipGetLastError());
+  dev_pass_ran_ = true;
+  stats.rule_apps += dp.apps;
+  if (timing) {
+    HIP_CHECK(hipEventRecord(t1, stream_));
+    pending_.push_back(TimedLaunch{t0, t1, dp.apps, (outside ? 4 : 0) + 16 * li, dp.apps_first, dp.apps_count});
+    stats.rule_launches += 1;
+  }
+  return true;
+}
+
 void Parser::launch_plans(Batch& B, int32_t li) {
   const Batch::LayerJobs* lj = nullptr;
   for (const auto& x : B.layer_jobs)
@@ -2443,6 +2856,78 @@ void Parser::launch_plans(Batch& B, int32_t li) {
   HIP_CHECK(hipGetLastError());
 }
 
+// Arguments of one slice of a (reduce group, span): each family's rule-kernel arguments and
+// partial rows (laid out from `base`), and the reduce's arguments. Returns the partial floats used.
+size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span, bool use_masks,
+                                const PlannedSlice* ps, int32_t k0, int32_t kc, float* base, GroupSlice* out) {
+  Layer& L = *layers_[li];
+  const DevReduce& R = L.red[rid];
+  const Grammar& g = L.g;
+  const size_t nfam = R.families.size();
+  GroupSlice& gs = *out;
+  gs = GroupSlice{};
+  gs.nfam = (int32_t)nfam;
+  RedArgs& ra = gs.ra;
+  size_t off = 0;
+  for (size_t fi = 0; fi < nfam; ++fi) {
+    const int fid = R.families[fi];
+    const DevFamily& F = L.fam[fid];
+    const int64_t nitems = family_items(B, li, fid, span, k0, kc);
+    if (use_masks) {
+      ra.prefix[fi] = ps->pfs[fi].pre;
+      ra.posof[fi] = ps->pfs[fi].posof;
+      ra.uniform_count[fi] = 1;
+      ra.act[fi] = ps->pfs[fi].act;
+      ra.block_of_target[fi] = F.unary ? nullptr : F.d_block_of_target;
+      ra.act_stride = kc;
+    } else if (uniform_kind(F.kind)) {
+      ra.prefix[fi] = nullptr;
+      ra.uniform_count[fi] = F.kind == 0 ? span - 1 : 1;
+    } else {
+      const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
+      ra.prefix[fi] = B.d_prefix + po + k0;
+      ra.uniform_count[fi] = 1;
+    }
+    int64_t row_items = nitems;  // partial values per slot row
+    if (!use_masks && uniform_kind(F.kind)) row_items = uniform_items(kc, ra.uniform_count[fi]);
+    if (!use_masks && pairs_splits(L, F, span)) {
+      ra.uniform_count[fi] = span / 2;
+      row_items = uniform_items(kc, span / 2);
+    }
+    const int32_t pitch = (int32_t)((row_items + 255) / 256 * 256);
+    ra.partial[fi] = base + off;
+    ra.pitch[fi] = pitch;
+    gs.fid[fi] = fid;
+    gs.partial[fi] = base + off;
+    gs.apps_slot[fi] = ps ? ps->apps_slot[fi] : -1;
+    gs.valid[fi] = family_args(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? ps->pfs[fi].pre : nullptr,
+                               use_masks ? ps->pfs[fi].items : nullptr, use_masks ? ps->pfs[fi].act : nullptr,
+                               &gs.fa[fi], &gs.grid[fi], &gs.lds[fi]);
+    off += (size_t)pitch * F.num_slots;
+  }
+  float* inside = (float*)inside_.p;
+  float* outside = (float*)outside_.p;
+  const DevFamily& F0 = L.fam[R.families[0]];
+  ra.chT = F0.t_half >= 2 ? outside : inside;
+  ra.t_half = F0.t_half;
+  ra.perm = ps ? ps->d_perm : nullptr;
+  ra.cell_offsets = B.d_cell_offsets;
+  ra.lengths = B.d_lengths;
+  ra.cell_list = B.d_cells + B.span_start[span] + k0;
+  ra.off = R.d_off;
+  ra.fam = R.d_fam;
+  ra.slot = R.d_slot;
+  ra.num_cells = kc;
+  ra.cell_size = g.cell_size;
+  ra.n_t = R.n_t;
+  ra.span = span;
+  ra.root_init = (rid == kROutBin);
+  ra.root = g.root;
+  ra.one = g.semiring == kReal ? 1.0f : 0.0f;
+  gs.rgrid = dim3((kc + 63) / 64, (R.n_t + reduce_targets_ - 1) / reduce_targets_);
+  return off;
+}
+
 void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use_masks,
                        const std::vector<PlannedSlice>* planned) {
   Layer& L = *layers_[li];
@@ -2451,61 +2936,21 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
   const int32_t ncells = B.span_count[span];
   if (ncells == 0) return;
   const size_t nfam = R.families.size();
-  auto slice_items = [&](int fid, int k0, int kc) -> int64_t { return family_items(B, li, fid, span, k0, kc); };
   const auto slices = group_slices(B, li, rid, span);
   for (size_t si = 0; si < slices.size(); ++si) {
     const int32_t k0 = slices[si].first, kc = slices[si].second;
-    const int32_t* cells = B.d_cells + B.span_start[span] + k0;
     const PlannedSlice* ps = nullptr;
     if (use_masks) {
       if (planned == nullptr || planned->size() != slices.size()) throw std::logic_error("masked group without its plan");
       ps = &(*planned)[si];
     }
-    const std::vector<PlanPtrs> empty;
-    const std::vector<PlanPtrs>& pfs = ps ? ps->pfs : empty;
-    const std::vector<int32_t> no_slots(nfam, -1);
-    const std::vector<int32_t>& apps_slot = ps ? ps->apps_slot : no_slots;
-    int32_t* d_perm = ps ? ps->d_perm : nullptr;
-
-    RedArgs ra{};
-    float* base = (float*)partial_.p;
-    size_t off = 0;
-    std::vector<float*> fam_partial(nfam);
-    for (size_t fi = 0; fi < nfam; ++fi) {
-      const int fid = R.families[fi];
-      const DevFamily& F = L.fam[fid];
-      const int64_t nitems = slice_items(fid, k0, kc);
-      if (use_masks) {
-        ra.prefix[fi] = pfs[fi].pre;
-        ra.posof[fi] = pfs[fi].posof;
-        ra.uniform_count[fi] = 1;
-        ra.act[fi] = pfs[fi].act;
-        ra.block_of_target[fi] = F.unary ? nullptr : F.d_block_of_target;
-        ra.act_stride = kc;
-      } else if (uniform_kind(F.kind)) {
-        ra.prefix[fi] = nullptr;
-        ra.uniform_count[fi] = F.kind == 0 ? span - 1 : 1;
-      } else {
-        const int64_t po = B.prefix_off[(size_t)span * kNumFamilies + fid];
-        ra.prefix[fi] = B.d_prefix + po + k0;
-        ra.uniform_count[fi] = 1;
-      }
-      int64_t row_items = nitems;  // partial values per slot row
-      if (!use_masks && uniform_kind(F.kind)) row_items = uniform_items(kc, ra.uniform_count[fi]);
-      if (!use_masks && pairs_splits(L, F, span)) {
-        ra.uniform_count[fi] = span / 2;
-        row_items = uniform_items(kc, span / 2);
-      }
-      const int32_t pitch = (int32_t)((row_items + 255) / 256 * 256);
-      ra.partial[fi] = base + off;
-      fam_partial[fi] = base + off;
-      ra.pitch[fi] = pitch;
-      if (!persistent)
-        launch_family(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? pfs[fi].pre : nullptr,
-                      use_masks ? pfs[fi].items : nullptr, use_masks ? pfs[fi].act : nullptr, apps_slot[fi]);
-      off += (size_t)pitch * F.num_slots;
-    }
-    if (persistent) {
+    GroupSlice gs;
+    group_slice_args(B, li, rid, span, use_masks, ps, k0, kc, (float*)partial_.p, &gs);
+    if (!persistent) {
+      for (size_t fi = 0; fi < nfam; ++fi)
+        launch_family(B, li, gs.fid[fi], span, use_masks, k0, kc, gs.partial[fi], use_masks ? ps->pfs[fi].pre : nullptr,
+                      use_masks ? ps->pfs[fi].items : nullptr, use_masks ? ps->pfs[fi].act : nullptr, gs.apps_slot[fi]);
+    } else {
       // One queue launch per layout among the group's families (unary, block-major, plain).
       for (int layout = 0; layout < 3; ++layout) {
         std::vector<int32_t> fids, slots;
@@ -2517,38 +2962,25 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
           const int lay = F.unary ? 0 : (F.skip_layout ? 1 : 2);
           if (lay != layout) continue;
           fids.push_back(R.families[fi]);
-          parts.push_back(fam_partial[fi]);
-          pcs.push_back(use_masks ? pfs[fi].pre : nullptr);
-          pis.push_back(use_masks ? pfs[fi].items : nullptr);
-          pas.push_back(use_masks ? pfs[fi].act : nullptr);
-          slots.push_back(apps_slot[fi]);
+          parts.push_back(gs.partial[fi]);
+          pcs.push_back(use_masks ? ps->pfs[fi].pre : nullptr);
+          pis.push_back(use_masks ? ps->pfs[fi].items : nullptr);
+          pas.push_back(use_masks ? ps->pfs[fi].act : nullptr);
+          slots.push_back(gs.apps_slot[fi]);
         }
         if (!fids.empty()) launch_queue(B, li, fids, span, use_masks, k0, kc, parts, pcs, pis, pas, slots);
       }
     }
-    float* inside = (float*)inside_.p;
-    float* outside = (float*)outside_.p;
-    const DevFamily& F0 = L.fam[R.families[0]];
-    ra.chT = F0.t_half >= 2 ? outside : inside;
-    ra.t_half = F0.t_half;
-    ra.perm = d_perm;
-    ra.cell_offsets = B.d_cell_offsets;
-    ra.lengths = B.d_lengths;
-    ra.cell_list = cells;
-    ra.off = R.d_off;
-    ra.fam = R.d_fam;
-    ra.slot = R.d_slot;
-    ra.num_cells = kc;
-    ra.cell_size = g.cell_size;
-    ra.n_t = R.n_t;
-    ra.span = span;
-    ra.root_init = (rid == kROutBin);
-    ra.root = g.root;
-    ra.one = g.semiring == kReal ? 1.0f : 0.0f;
-    dim3 grid((kc + 63) / 64, (R.n_t + kReduceTargets - 1) / kReduceTargets);
     launch_semi(g.semiring, [&](auto semi) {
       constexpr int S = decltype(semi)::value;
-      hipLaunchKernelGGL((k_reduce<S>), grid, dim3(64 * kReduceTargets), 0, stream_, ra);
+      const dim3 blk(64 * reduce_targets_);
+      switch (reduce_targets_) {
+        case 1: hipLaunchKernelGGL((k_reduce<S, 1>), gs.rgrid, blk, 0, stream_, gs.ra); break;
+        case 2: hipLaunchKernelGGL((k_reduce<S, 2>), gs.rgrid, blk, 0, stream_, gs.ra); break;
+        case 4: hipLaunchKernelGGL((k_reduce<S, 4>), gs.rgrid, blk, 0, stream_, gs.ra); break;
+        case 8: hipLaunchKernelGGL((k_reduce<S, 8>), gs.rgrid, blk, 0, stream_, gs.ra); break;
+        default: hipLaunchKernelGGL((k_reduce<S, kReduceTargets>), gs.rgrid, blk, 0, stream_, gs.ra); break;
+      }
     });
     HIP_CHECK(hipGetLastError());
   }
@@ -2575,7 +3007,7 @@ void Parser::inside_pass(Batch& B, int32_t li, bool use_masks) {
   HIP_CHECK(hipGetLastError());
   // A masked layer plans every masked pass of its own up front (the plans read only the masks).
   if (use_masks) launch_plans(B, li);
-  run_groups(B, li, false, use_masks);
+  if (!run_dev_pass(B, li, false, use_masks)) run_groups(B, li, false, use_masks);
 }
 
 void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
@@ -2584,7 +3016,7 @@ void Parser::outside_pass(Batch& B, int32_t li, bool use_masks) {
   const float zero = zero_of(g.semiring);
   const size_t nfl = (size_t)B.num_cells * g.cell_size;
   hipLaunchKernelGGL(k_fill, dim3(2048), dim3(256), 0, stream_, (float*)outside_.p, nfl, zero);
-  run_groups(B, li, true, use_masks);
+  if (!run_dev_pass(B, li, true, use_masks)) run_groups(B, li, true, use_masks);
 }
 
 // Runs a pass's (group, span) list in order, each masked group with its slices' plans.
@@ -2697,11 +3129,18 @@ void Parser::run_batch(const int32_t* words, const int32_t* offsets, int32_t fir
   ensure(partial_, partial_bytes);
   // Mask plans of every masked pass of the call, laid out now so that the job table travels with
   // the batch's other uploads (and a captured graph replays it unchanged).
-  prepare_plans(B, pass_schedule(what, chart_layer));
+  const auto schedule = pass_schedule(what, chart_layer);
+  prepare_plans(B, schedule);
   if (!B.h_jobs.empty()) {
     HIP_CHECK(hipMemcpyAsync(jobs_.p, B.h_jobs.data(), B.h_jobs.size() * sizeof(PlanJob), hipMemcpyHostToDevice,
                              stream_));
     B.d_jobs = (PlanJob*)jobs_.p;
+  }
+  prepare_dev_passes(B, schedule);
+  if (!B.h_pass_blob.empty()) {
+    ensure(pass_, B.h_pass_blob.size());
+    ensure(pcnt_, (size_t)B.num_pass_counters * 4);
+    HIP_CHECK(hipMemcpyAsync(pass_.p, B.h_pass_blob.data(), B.h_pass_blob.size(), hipMemcpyHostToDevice, stream_));
   }
   // Plan counters: one double per plan job. Queue heads: one per persistent launch, at most three
   // layouts per (layer, pass, group, slice); sized with room for eight slices per group.
@@ -2848,6 +3287,9 @@ void Parser::reset_counters(Batch& B) {
   apps_next_ = (int32_t)B.h_jobs.size();  // every plan job owns one counter
   queue_next_ = 0;
   B.next_pass = 0;
+  B.next_dev = 0;
+  if (B.num_pass_counters > 64)  // the error flag of the device-scheduled passes
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream_, (float*)pcnt_.p, (size_t)64, 0.0f);
 }
 
 // Device work of CLParser.parse on one batch: every layer's passes, the masks between layers and
@@ -3095,7 +3537,7 @@ void Parser::parse(const int32_t* words, const int32_t* offsets, int32_t n,
 
 void Parser::reset_arena() {
   for (Buf* b : {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_, &trees_, &mbr_,
-                 &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_})
+                 &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_, &pass_, &pcnt_})
     *b = Buf{};
   arena_.used = 0;
 }
@@ -3117,7 +3559,7 @@ void Parser::parse_noalloc(const int32_t* words, const int32_t* offsets, int32_t
                              "parse holds at most " + std::to_string(kNoAllocScratchBytes));
   // Swap the parser's own workspace out for the scratch array, and back afterwards.
   Buf* bufs[] = {&inside_, &outside_, &in_scale_, &out_scale_, &masks_, &partial_, &plan_, &tags_, &trees_, &mbr_,
-                 &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_};
+                 &pos_scale_, &aux_, &mplan_, &apps_, &queue_, &jobs_, &pass_, &pcnt_};
   std::vector<Buf> saved;
   for (Buf* b : bufs) saved.push_back(*b);
   const bool saved_graphs = use_graphs, saved_timing = timing;

@@ -61,6 +61,7 @@ struct FamArgs {
   int32_t row_stride;  // floats per LDS row
   int32_t num_chunks;  // operand chunks of the family (grid y of the plain launch)
   int32_t pair_splits; // uniform NN: items 2m, 2m+1 are splits 2p, 2p+1 of one cell (pair m)
+  int32_t skip_layout; // block-major rule stream (masked layers with several blocks)
 };
 
 // Arguments of the persistent rule kernel: the families of one span that share a layout.
@@ -97,6 +98,53 @@ struct RedArgs {
   int32_t num_cells, cell_size, n_t, t_half, span;
   int32_t root_init, root;
   float one;
+};
+
+// Device-scheduled pass (k_pass): every (reduce group, span) of one pass becomes two stages, its
+// rule families then its reduce, in schedule order; one persistent launch pulls units (a rule tile x
+// chunk x piece, or a reduce block of 64 cells x 16 targets) of all stages from one device queue in
+// that order. Dependencies are tracked per 64-cell block of a stage's cells with device counters:
+//   rc[b]  rule units of the group done that touched block b
+//   dc[b]  reduce units of the group done on block b (complete at `gy`, one per 16 targets)
+//   rtot, dtot  totals of the group's rule and reduce units
+// A reduce unit of block b waits until rc[b] reaches the rule units that touch b (host-counted); a
+// rule tile waits until the previous group's reduce blocks that hold its sentences' cells are
+// complete (the previous group finished its chain of earlier spans for those sentences), and until
+// the group two back has finished reducing (its half of the partial buffer is reused). Units only
+// wait on units earlier in the queue, and every unit in the queue belongs to a running workgroup, so
+// the waits always end; each is also bounded, and a wait that times out sets an error flag that
+// makes every later wait pass at once (the host reports it).
+constexpr int kMaxPassFamilies = 5;
+constexpr int kPassSpinLimit = 1 << 20;  // polls of a counter before a wait gives up (about a second)
+struct PassStage {
+  int32_t kind;          // 0: rule families, 1: reduce
+  int32_t first_unit, num_units;
+  int32_t cnt, nb;       // this group's counters: rc [cnt, cnt + nb), dc [cnt + nb, cnt + 2 nb), rtot, dtot
+  int32_t gy;            // reduce units per block (target groups)
+  int32_t rule_units;    // units of the group's rule stage
+  int32_t dep_cnt, dep_nb, dep_gy, dep_units;  // previous group's counters (dep_cnt < 0: none)
+  int32_t war_cnt, war_nb, war_units;          // the group two back (war_cnt < 0: none)
+  int32_t exp_off;       // reduce: expected rule units per block at expected[exp_off + b]; < 0: wait on rtot
+  int32_t nfam;
+  int32_t fam_units[kMaxPassFamilies];  // rule units before family f
+  int32_t fam_tiles[kMaxPassFamilies];
+  int32_t tile_base[kMaxPassFamilies];  // family f's first entry in the tile table
+  FamArgs fam[kMaxPassFamilies];
+  RedArgs red;
+};
+// Per rule tile: output blocks [x, y] (x < 0: none, the group's reduce waits on rtot) and the
+// previous group's blocks it waits on [z, w] (z == -2: all of them, by dtot; z > w: none).
+struct PassTile {
+  int32_t out_lo, out_hi, dep_lo, dep_hi;
+};
+struct PassArgs {
+  const PassStage* stages;
+  const PassTile* tiles;
+  const int32_t* expected;
+  int32_t* counters;  // zero before the launch
+  int32_t* queue;     // queue head, zero before the launch
+  int32_t* error;     // set when a wait timed out
+  int32_t num_stages, total_units;
 };
 
 // Mask plans. A plan job is one family of one partial-buffer slice of one (reduce group, span) of

@@ -45,6 +45,17 @@ def test_parser_create_without_device_fails_cleanly(puck, wsj0):
         puck.CLParser([wsj0], mem_bytes=1 << 28)
 
 
+def test_rejected_configurations_fail_before_device_work(puck, wsj0):
+    # Checked before any HIP call, so these run without a GPU.
+    with pytest.raises(puck.PuckError, match="device-scheduled passes are disabled"):
+        puck.CLParser([wsj0], mem_bytes=1 << 28, device_schedule=True)
+    with pytest.raises(puck.PuckError, match="device-scheduled passes are disabled"):
+        puck.BatchParser([wsj0], mem_bytes=1 << 28, device_schedule=True)
+    for rt in (0, 3, 32):
+        with pytest.raises(puck.PuckError, match="reduce_targets must be 1, 2, 4, 8 or 16"):
+            puck.BatchParser([wsj0], mem_bytes=1 << 28, reduce_targets=rt)
+
+
 def test_sampler_is_seeded_and_respects_lengths(wsj1):
     w1, o1 = wsj1.sample(seed=42, n=50, mode=0, min_len=5, max_len=20)
     w2, o2 = wsj1.sample(seed=42, n=50, mode=0, min_len=5, max_len=20)

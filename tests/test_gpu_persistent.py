@@ -57,6 +57,29 @@ def test_launch_shapes_give_identical_trees(puck, wsj0, wsj1):
             assert np.array_equal(a, b), f"persistent={persistent} target={target} wgs={wgs} sentence {i}"
 
 
+def test_reduce_workgroup_sizes_give_identical_charts(puck, wsj0, wsj1, wsj1_real):
+    """The reduce kernel with 1-16 targets (waves) per workgroup folds each (cell, target) in the
+    same order: inside and outside charts, masks, trees and Real partitions are bit-identical."""
+    sents = wsj1.sample(seed=75, n=30, mode=0, min_len=2, max_len=28)
+    ref = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30, persistent=True)
+    ref_ch = ref.debug_charts(sents, layer=1, with_outside=True)
+    ref_trees, ref_masks = ref.parse(sents), ref.masks(sents)
+    ref_r = puck.CLParser([wsj0, wsj1_real], mem_bytes=2 << 30, persistent=True)
+    ref_parts = ref_r.partitions(sents)
+    for rt in (1, 2, 8, 16):
+        p = puck.CLParser([wsj0, wsj1], mem_bytes=2 << 30)
+        p.configure(True, reduce_targets=rt)
+        ch = p.debug_charts(sents, layer=1, with_outside=True)
+        assert np.array_equal(ch["inside"], ref_ch["inside"]), f"reduce_targets={rt} inside"
+        assert np.array_equal(ch["outside"], ref_ch["outside"]), f"reduce_targets={rt} outside"
+        assert np.array_equal(p.masks(sents), ref_masks), f"reduce_targets={rt} masks"
+        for i, (a, b) in enumerate(zip(p.parse(sents), ref_trees)):
+            assert np.array_equal(a, b), f"reduce_targets={rt} sentence {i}"
+        pr = puck.CLParser([wsj0, wsj1_real], mem_bytes=2 << 30)
+        pr.configure(True, reduce_targets=rt)
+        assert np.array_equal(pr.partitions(sents), ref_parts), f"reduce_targets={rt} partitions"
+
+
 def test_wsj6_shape_block_major_streams(puck, wsj1):
     """The synthetic wsj_6-shape grammar has 15-18 blocks per family: every fine binary family
     uses the block-major (skipping) layout in one shared queue launch."""
