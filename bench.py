@@ -125,7 +125,7 @@ def plain_vs_batched(puck, layers, words, offsets, steps, cfg, label, outside_fi
     del p
     q = puck.CLParser(layers, device=0, mem_bytes=64 << 30, items_per_lane=cfg["items_per_lane"], persistent=True,
                       outside_final=outside_final)
-    q.configure(True, piece_target=cfg["piece_target"])
+    q.configure(True, piece_target=cfg["piece_target"], reduce_targets=cfg["reduce_targets"])
     _, launches_q = timed_parses(q, words, offsets, 1)
     del q
     bp = puck.BatchParser(layers, mem_bytes=96 << 30, **cfg)
@@ -271,15 +271,22 @@ def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
     return out
 
 
+# Autotuned configurations (python -m puck_amd.autotune): the mixed-length coarse-to-fine workload's
+# serves every coarse-to-fine side workload; the exhaustive insideOutside workload's the headline.
 TUNED = os.path.join(ROOT, "puck_amd", "tuning", "batched_wsj6.json")
+TUNED_IO = os.path.join(ROOT, "puck_amd", "tuning", "batched_wsj6_inside_outside.json")
 
 
-def batched_config(args):
-    """The autotuned configuration (puck_amd/autotune.py) when one is committed, else defaults."""
+def batched_config(args, path=TUNED):
+    """The autotuned configuration for a workload when one is committed (else the mixed workload's,
+    else the defaults)."""
     from puck_amd import autotune
     cfg = dict(autotune.DEFAULTS)
-    if os.path.exists(TUNED) and not args.no_tuned:
-        cfg.update(autotune.load(TUNED)["best"])
+    if not args.no_tuned:
+        for p in dict.fromkeys((path, TUNED)):
+            if os.path.exists(p):
+                cfg.update(autotune.load(p)["best"])
+                break
     return cfg
 
 
@@ -335,7 +342,7 @@ def main():
     words, offsets = g.sample(seed=args.seed, n=args.sentences, mode=1, min_len=1, max_len=40)
     setup_s = time.time() - t0
     lens = np.diff(offsets)
-    cfg = batched_config(args)
+    cfg = batched_config(args, TUNED_IO)
     print(f"[bench] setup {setup_s:.1f}s; {len(lens)} sentences, mean length {lens.mean():.1f}, "
           f"max {lens.max()}; batched config {cfg}", file=sys.stderr, flush=True)
 
@@ -352,7 +359,7 @@ def main():
     # families: family ids 0-11 of layer 0).
     pers = puck.CLParser([g], device=0, mem_bytes=64 << 30, items_per_lane=cfg["items_per_lane"], persistent=True,
                          outside_final=True)
-    pers.configure(True, piece_target=cfg["piece_target"])
+    pers.configure(True, piece_target=cfg["piece_target"], reduce_targets=cfg["reduce_targets"])
     pers.parse((words, offsets))
     kern_ms, kern_apps, launches = 0.0, 0.0, 0
     for i in range(args.steps):

@@ -39,8 +39,8 @@ SPACE = dict(
 )
 
 
-def _measure(puck, layers, sents, cfg, mem_bytes, reps):
-    bp = puck.BatchParser(layers, mem_bytes=mem_bytes, **cfg)
+def _measure(puck, layers, sents, cfg, mem_bytes, reps, fixed):
+    bp = puck.BatchParser(layers, mem_bytes=mem_bytes, **cfg, **fixed)
     trees = None
     # First call runs eagerly, the second captures the bucket graphs, later calls replay them.
     for _ in range(2):
@@ -54,10 +54,12 @@ def _measure(puck, layers, sents, cfg, mem_bytes, reps):
     return best, trees, st
 
 
-def tune(puck, layers, sentences, mem_bytes=96 << 30, rounds=2, reps=2, log=None):
+def tune(puck, layers, sentences, mem_bytes=96 << 30, rounds=2, reps=2, log=None, fixed=None):
     """Returns {"best": cfg, "best_ms": t, "plain_ms": t0, "table": [...]} for the workload sample
-    `sentences` ((words, offsets) or a list of word-id arrays)."""
-    plain = puck.CLParser(layers, mem_bytes=mem_bytes)
+    `sentences` ((words, offsets) or a list of word-id arrays). `fixed`: options of the workload
+    itself, passed to every parser and not tuned (outside_final)."""
+    fixed = dict(fixed or {})
+    plain = puck.CLParser(layers, mem_bytes=mem_bytes, **fixed)
     ref = plain.parse(sentences)
     plain.parse(sentences)
     plain_ms = plain.stats()["device_ms"]
@@ -70,7 +72,7 @@ def tune(puck, layers, sentences, mem_bytes=96 << 30, rounds=2, reps=2, log=None
         if key in seen:
             return seen[key]
         t0 = time.time()
-        ms, trees, st = _measure(puck, layers, sentences, cfg, mem_bytes, reps)
+        ms, trees, st = _measure(puck, layers, sentences, cfg, mem_bytes, reps, fixed)
         same = len(trees) == len(ref) and all(np.array_equal(a, b) for a, b in zip(trees, ref))
         row = dict(cfg, device_ms=round(ms, 3), buckets=int(st["buckets"]), same_trees=bool(same),
                    wall_s=round(time.time() - t0, 2))
@@ -110,25 +112,39 @@ def load(path):
 
 
 def main(argv=None):
-    """python -m puck_amd.autotune --out FILE: tunes the batched engine on a seeded sample of the
-    mixed-length coarse-to-fine workload (wsj_1 -> synthetic wsj_6 shape, 70% log-normal 1-40 +
-    30% uniform 41-80) and writes the result as JSON."""
+    """python -m puck_amd.autotune --out FILE [--workload mixed|inside_outside]: tunes the batched
+    engine on a seeded sample of a workload and writes the result as JSON.
+
+      mixed           the mixed-length coarse-to-fine workload (wsj_1 -> synthetic wsj_6 shape,
+                      Viterbi, 70% log-normal 1-40 + 30% uniform 41-80; default 4,000 sentences)
+      inside_outside  exhaustive insideOutside + Viterbi decode on the synthetic wsj_6 shape,
+                      log-normal lengths 1-40 (default 1,000 sentences)
+    """
     import argparse
     import sys
     import puck_amd as puck
     ap = argparse.ArgumentParser(prog="puck_amd.autotune")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sentences", type=int, default=4000)
+    ap.add_argument("--workload", choices=("mixed", "inside_outside"), default="mixed")
+    ap.add_argument("--sentences", type=int, default=0, help="sample size (default 4000 mixed, 1000 inside_outside)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--rounds", type=int, default=2)
     args = ap.parse_args(argv)
     g6 = puck.Grammar.load_text(puck.synth_wsj6_cached(seed=6), puck.VITERBI)
-    g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
-    sents = g6.sample(seed=args.seed, n=args.sentences, mode=2, min_len=1, max_len=80)
-    res = tune(puck, [g1, g6], sents, rounds=args.rounds,
-               log=lambda row: print(f"[autotune] {row}", file=sys.stderr, flush=True))
-    workload = (f"wsj_1 -> synthetic wsj_6-shape c2f Viterbi, {args.sentences} sentences (seed {args.seed}), "
-                "70% log-normal 1-40 + 30% uniform 41-80")
+    log = lambda row: print(f"[autotune] {row}", file=sys.stderr, flush=True)  # noqa: E731
+    if args.workload == "mixed":
+        n = args.sentences or 4000
+        g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
+        sents = g6.sample(seed=args.seed, n=n, mode=2, min_len=1, max_len=80)
+        res = tune(puck, [g1, g6], sents, rounds=args.rounds, log=log)
+        workload = (f"wsj_1 -> synthetic wsj_6-shape c2f Viterbi, {n} sentences (seed {args.seed}), "
+                    "70% log-normal 1-40 + 30% uniform 41-80")
+    else:
+        n = args.sentences or 1000
+        sents = g6.sample(seed=args.seed, n=n, mode=1, min_len=1, max_len=40)
+        res = tune(puck, [g6], sents, rounds=args.rounds, log=log, fixed=dict(outside_final=True))
+        workload = (f"synthetic wsj_6-shape exhaustive insideOutside + Viterbi decode, {n} sentences "
+                    f"(seed {args.seed}), log-normal lengths 1-40")
     save(args.out, res, workload)
     print(json.dumps({"best": res["best"], "best_ms": res["best_ms"], "plain_ms": res["plain_ms"]}))
 

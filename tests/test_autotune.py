@@ -12,9 +12,11 @@ from puck_amd import autotune
 def fake_puck(cost, wrong=None):
     trees = [np.arange(7, dtype=np.int32).reshape(-1, 1)]
 
+    seen = []
+
     class CLParser:
-        def __init__(self, layers, mem_bytes):
-            pass
+        def __init__(self, layers, mem_bytes, **kw):
+            seen.append(("plain", kw))
 
         def parse(self, s):
             return trees
@@ -25,6 +27,7 @@ def fake_puck(cost, wrong=None):
     class BatchParser:
         def __init__(self, layers, mem_bytes, **cfg):
             self.cfg = cfg
+            seen.append(("batched", cfg))
 
         def parse(self, s):
             if wrong is not None and all(self.cfg.get(k) == v for k, v in wrong.items()):
@@ -34,7 +37,7 @@ def fake_puck(cost, wrong=None):
         def stats(self):
             return {"device_ms": cost(self.cfg), "buckets": 3}
 
-    return types.SimpleNamespace(CLParser=CLParser, BatchParser=BatchParser)
+    return types.SimpleNamespace(CLParser=CLParser, BatchParser=BatchParser, seen=seen)
 
 
 def test_coordinate_descent_finds_separable_minimum():
@@ -60,3 +63,11 @@ def test_save_and_load_roundtrip(tmp_path):
     autotune.save(str(path), res, "unit")
     back = autotune.load(str(path))
     assert back["best"]["streams"] == 1 and back["workload"] == "unit"
+
+
+def test_workload_options_reach_every_parser_and_are_not_tuned():
+    fake = fake_puck(lambda c: 10.0 + c["reduce_targets"])
+    res = autotune.tune(fake, [None], ([0], [0, 1]), fixed={"outside_final": True})
+    assert all(kw.get("outside_final") is True for _, kw in fake.seen)
+    assert "outside_final" not in res["best"]
+    assert res["best"]["reduce_targets"] == 1
