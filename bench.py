@@ -136,7 +136,7 @@ def plain_vs_batched(puck, layers, words, offsets, steps, cfg, label, outside_fi
            "value": round(len(lens) / (ms_b / 1e3), 1), "unit": "sentences/s", "ms_per_parse": round(ms_b, 2),
            "plain": round(len(lens) / (ms_p / 1e3), 1), "plain_ms_per_parse": round(ms_p, 2),
            "batched_vs_plain": round(ms_p / ms_b, 3), "trees_identical": same_trees(trees, ref),
-           "buckets": int(st["buckets"]), "graph_replays": int(st["graph_replays"]),
+           "buckets": int(st["buckets"]), "graph_replays": int(st["graph_replays"]), "batched": cfg,
            "roofline_plain": fine_roofline(launches, kernel_labels[0]),
            "roofline_batched": fine_roofline(launches_q, kernel_labels[1])}
     print(f"[bench] {label}: batched {out['value']} vs plain {out['plain']} sentences/s", file=sys.stderr, flush=True)
@@ -192,16 +192,19 @@ def serve(puck, layers, pool, rate, seconds, seed):
 SIDE = ("config1", "c2f_viterbi", "c2f_mbr", "c2f_reproject", "mixed_lengths", "serving")
 
 
-def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
+def side_workloads(puck, g6v, prefix, args, which=SIDE):
     """Config 1, coarse-to-fine, mixed-length and serving workloads, measured beside the headline,
-    with the oracle's CPU time on config 1 (in full) and config 3 (a stated sample)."""
+    with the oracle's CPU time on config 1 (in full) and config 3 (a stated sample). Each batched
+    workload runs its own autotuned configuration."""
     out = {}
+    cfg_io, cfg = batched_config(args, TUNED_IO), batched_config(args, TUNED_C2F)
+    cfg_mixed = batched_config(args, TUNED)
     g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
     words, offsets = g6v.sample(seed=3, n=args.c2f_sentences, mode=0, min_len=1, max_len=40)
     if "config1" in which:
         # Config 1: wsj_1, 200 PCFG sentences of 5-20 words, exhaustive insideOutside + Viterbi.
         w1, o1 = g1.sample(seed=1, n=200, mode=0, min_len=5, max_len=20)
-        c1 = dict(cfg, outside_final=True)
+        c1 = dict(cfg_io, outside_final=True)
         rec = plain_vs_batched(puck, [g1], w1, o1, 3, c1, "config 1: wsj_1 exhaustive insideOutside + Viterbi, "
                                "200 PCFG sentences 5-20", outside_final=True,
                                kernel_labels=("k_rules (inside binary)", "k_rules_q (inside binary, persistent)"))
@@ -257,7 +260,7 @@ def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
     if "mixed_lengths" in which:
         mw, mo = g6v.sample(seed=4, n=args.mixed_sentences, mode=2, min_len=1, max_len=80)
         out["mixed_lengths"] = plain_vs_batched(
-            puck, [g1, g6v], mw, mo, 1, cfg,
+            puck, [g1, g6v], mw, mo, 1, cfg_mixed,
             f"config 4: c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
     if "serving" in which:
         # Request lengths as in the coarse-to-fine workload: PCFG samples of length 1-40.
@@ -271,10 +274,12 @@ def side_workloads(puck, g6v, prefix, args, cfg, which=SIDE):
     return out
 
 
-# Autotuned configurations (python -m puck_amd.autotune): the mixed-length coarse-to-fine workload's
-# serves every coarse-to-fine side workload; the exhaustive insideOutside workload's the headline.
+# Autotuned configurations (python -m puck_amd.autotune), one per workload: the exhaustive
+# insideOutside workload's serves the headline and config 1, the coarse-to-fine workload's the
+# config-3 workloads, the mixed-length workload's config 4.
 TUNED = os.path.join(ROOT, "puck_amd", "tuning", "batched_wsj6.json")
 TUNED_IO = os.path.join(ROOT, "puck_amd", "tuning", "batched_wsj6_inside_outside.json")
+TUNED_C2F = os.path.join(ROOT, "puck_amd", "tuning", "batched_wsj6_c2f.json")
 
 
 def batched_config(args, path=TUNED):
@@ -337,7 +342,7 @@ def main():
         bad = [w for w in which if w not in SIDE]
         if bad:
             sys.exit(f"unknown workloads {bad}; choose from {SIDE}")
-        print(json.dumps(side_workloads(puck, g, prefix, args, batched_config(args), which)))
+        print(json.dumps(side_workloads(puck, g, prefix, args, which)))
         return
     words, offsets = g.sample(seed=args.seed, n=args.sentences, mode=1, min_len=1, max_len=40)
     setup_s = time.time() - t0
@@ -445,7 +450,7 @@ def main():
         line["trees_match_oracle"] = f"{k}/{k}" if ok else "MISMATCH"
         del layers, res
     if args.side:
-        line.update(side_workloads(puck, g, prefix, args, batched_config(args)))
+        line.update(side_workloads(puck, g, prefix, args))
     print(json.dumps(line))
 
 

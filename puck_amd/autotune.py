@@ -112,11 +112,13 @@ def load(path):
 
 
 def main(argv=None):
-    """python -m puck_amd.autotune --out FILE [--workload mixed|inside_outside]: tunes the batched
-    engine on a seeded sample of a workload and writes the result as JSON.
+    """python -m puck_amd.autotune --out FILE [--workload mixed|c2f|inside_outside]: tunes the
+    batched engine on a seeded sample of a workload and writes the result as JSON.
 
       mixed           the mixed-length coarse-to-fine workload (wsj_1 -> synthetic wsj_6 shape,
                       Viterbi, 70% log-normal 1-40 + 30% uniform 41-80; default 4,000 sentences)
+      c2f             coarse-to-fine Viterbi (wsj_1 -> synthetic wsj_6 shape) on PCFG samples of
+                      lengths 1-40 (default 5,000 sentences)
       inside_outside  exhaustive insideOutside + Viterbi decode on the synthetic wsj_6 shape,
                       log-normal lengths 1-40 (default 1,000 sentences)
     """
@@ -125,20 +127,22 @@ def main(argv=None):
     import puck_amd as puck
     ap = argparse.ArgumentParser(prog="puck_amd.autotune")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--workload", choices=("mixed", "inside_outside"), default="mixed")
-    ap.add_argument("--sentences", type=int, default=0, help="sample size (default 4000 mixed, 1000 inside_outside)")
+    ap.add_argument("--workload", choices=("mixed", "c2f", "inside_outside"), default="mixed")
+    ap.add_argument("--sentences", type=int, default=0,
+                    help="sample size (default 4000 mixed, 5000 c2f, 1000 inside_outside)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--rounds", type=int, default=2)
     args = ap.parse_args(argv)
     g6 = puck.Grammar.load_text(puck.synth_wsj6_cached(seed=6), puck.VITERBI)
     log = lambda row: print(f"[autotune] {row}", file=sys.stderr, flush=True)  # noqa: E731
-    if args.workload == "mixed":
-        n = args.sentences or 4000
+    if args.workload in ("mixed", "c2f"):
+        mixed = args.workload == "mixed"
+        n = args.sentences or (4000 if mixed else 5000)
         g1 = puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.VITERBI)
-        sents = g6.sample(seed=args.seed, n=n, mode=2, min_len=1, max_len=80)
+        sents = g6.sample(seed=args.seed, n=n, mode=2 if mixed else 0, min_len=1, max_len=80 if mixed else 40)
         res = tune(puck, [g1, g6], sents, rounds=args.rounds, log=log)
         workload = (f"wsj_1 -> synthetic wsj_6-shape c2f Viterbi, {n} sentences (seed {args.seed}), "
-                    "70% log-normal 1-40 + 30% uniform 41-80")
+                    + ("70% log-normal 1-40 + 30% uniform 41-80" if mixed else "PCFG samples of lengths 1-40"))
     else:
         n = args.sentences or 1000
         sents = g6.sample(seed=args.seed, n=n, mode=1, min_len=1, max_len=40)
