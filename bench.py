@@ -189,7 +189,7 @@ def serve(puck, layers, pool, rate, seconds, seed):
             "mean_batch": round(st["mean_batch"], 1)}
 
 
-SIDE = ("config1", "c2f_viterbi", "c2f_mbr", "c2f_reproject", "mixed_lengths", "serving")
+SIDE = ("config1", "c2f_viterbi", "c2f_mbr", "c2f_reproject", "mixed_lengths", "serving", "serving_short")
 
 
 def side_workloads(puck, g6v, prefix, args, which=SIDE):
@@ -263,14 +263,25 @@ def side_workloads(puck, g6v, prefix, args, which=SIDE):
             puck, [g1, g6v], mw, mo, 1, cfg_mixed,
             f"config 4: c2f Viterbi, {args.mixed_sentences} sentences, 70% log-normal 1-40 + 30% uniform 41-80")
     if "serving" in which:
-        # Request lengths as in the coarse-to-fine workload: PCFG samples of length 1-40.
-        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
+        # Config 5: request lengths as in config 4 (70% log-normal 1-40, 30% uniform 41-80).
+        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=2, min_len=1, max_len=80)
         pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
         out["serving"] = []
         for rate in (200, 1000, 5000):
             r = serve(puck, [g1, g6v], pool, rate, args.serve_seconds, seed=rate)
+            r["lengths"] = "config 4: 70% log-normal 1-40 + 30% uniform 41-80"
             out["serving"].append(r)
             print(f"[bench] serving {r}", file=sys.stderr, flush=True)
+    if "serving_short" in which:
+        # The same server on requests of the coarse-to-fine workload's lengths (PCFG samples 1-40).
+        pool_w, pool_o = g6v.sample(seed=5, n=20000, mode=0, min_len=1, max_len=40)
+        pool = [np.ascontiguousarray(pool_w[pool_o[i]:pool_o[i + 1]]) for i in range(len(pool_o) - 1)]
+        out["serving_short"] = []
+        for rate in (200, 1000, 5000):
+            r = serve(puck, [g1, g6v], pool, rate, args.serve_short_seconds, seed=rate)
+            r["lengths"] = "PCFG samples 1-40"
+            out["serving_short"].append(r)
+            print(f"[bench] serving_short {r}", file=sys.stderr, flush=True)
     return out
 
 
@@ -315,6 +326,8 @@ def main():
     ap.add_argument("--mixed-sentences", type=int, default=20000)
     ap.add_argument("--no-tuned", action="store_true", help="use the default batched configuration")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="seconds of Poisson arrivals per rate")
+    ap.add_argument("--serve-short-seconds", type=float, default=10.0,
+                    help="seconds per rate of the serving run on requests of length 1-40")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (counter calibration)")
     args = ap.parse_args()

@@ -232,8 +232,11 @@ int32_t puck_batch_stats(puck_batch_t* b, puck_batch_stats_t* out);
 
 /* ---- serving (replaces BatchFunctionAnnotatorService, AnnotatorService.scala:29-78) ----
  * Continuous batching: up to max_inflight batches run concurrently (one HIP stream each); a free
- * slot admits queued requests in arrival order while their chart cells fit its memory budget.
- * flags: PUCK_FLAG_ONE_ITEM_PER_LANE and PUCK_FLAG_PERSISTENT as for puck_parser_create. */
+ * slot admits queued requests in arrival order while the device memory their batch needs fits what
+ * the slot holds plus the device's free memory. A slot parses its batch in two length classes, at
+ * most 40 words first, and publishes each class's results when it is done; `batches` counts these
+ * parse calls. flags: PUCK_FLAG_ONE_ITEM_PER_LANE and PUCK_FLAG_PERSISTENT as for
+ * puck_parser_create. */
 typedef struct puck_server puck_server_t;
 typedef struct {
   int64_t completed, batches;

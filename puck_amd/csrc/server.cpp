@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <numeric>
 #include <stdexcept>
 
 namespace puck {
@@ -112,42 +113,59 @@ void Server::worker(int32_t slot) {
       claimed_ += growth;
       if (!queue_.empty()) cv_work_.notify_one();
     }
-    std::vector<int32_t> words, offsets{0};
-    for (auto& r : batch) {
-      words.insert(words.end(), r.words.begin(), r.words.end());
-      offsets.push_back((int32_t)words.size());
-    }
-    std::vector<std::vector<TreeNode>> vit;
-    std::vector<std::vector<MbrNode>> mbr;
-    int32_t status = 0;
-    std::string err;
-    try {
-      if (mbr_final_) p.parse(words.data(), offsets.data(), (int32_t)batch.size(), nullptr, &mbr);
-      else p.parse(words.data(), offsets.data(), (int32_t)batch.size(), &vit, nullptr);
-    } catch (const std::exception& e) {
-      status = -5;
-      err = e.what();
-    }
-    const auto now = std::chrono::steady_clock::now();
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      claimed_ -= growth;  // allocated by now (or failed)
-      for (size_t i = 0; i < batch.size(); ++i) {
-        Done d;
-        d.status = status;
-        d.error = err;
-        if (status == 0) {
-          if (mbr_final_) d.mbr = std::move(mbr[i]);
-          else d.nodes = std::move(vit[i]);
-        }
-        done_[batch[i].ticket] = std::move(d);
-        record_latency(std::chrono::duration<double, std::milli>(now - batch[i].submitted).count());
+    // Length classes, shortest first: sentences of at most kSplitLength words are parsed (and their
+    // results published) before the longer ones, so a short request does not wait for the span
+    // loop of a long one admitted with it. Within a class, arrival order.
+    std::vector<size_t> order(batch.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_partition(order.begin(), order.end(),
+                          [&](size_t i) { return (int32_t)batch[i].words.size() <= kSplitLength; });
+    size_t first = 0;
+    while (first < order.size()) {
+      const bool short_class = (int32_t)batch[order[first]].words.size() <= kSplitLength;
+      size_t last = first;
+      while (last < order.size() && ((int32_t)batch[order[last]].words.size() <= kSplitLength) == short_class) ++last;
+      std::vector<int32_t> words, offsets{0};
+      for (size_t k = first; k < last; ++k) {
+        const Request& r = batch[order[k]];
+        words.insert(words.end(), r.words.begin(), r.words.end());
+        offsets.push_back((int32_t)words.size());
       }
-      outstanding_ -= (int64_t)batch.size();
-      ++batches_;
-      batched_requests_ += (int64_t)batch.size();
+      const int32_t n = (int32_t)(last - first);
+      std::vector<std::vector<TreeNode>> vit;
+      std::vector<std::vector<MbrNode>> mbr;
+      int32_t status = 0;
+      std::string err;
+      try {
+        if (mbr_final_) p.parse(words.data(), offsets.data(), n, nullptr, &mbr);
+        else p.parse(words.data(), offsets.data(), n, &vit, nullptr);
+      } catch (const std::exception& e) {
+        status = -5;
+        err = e.what();
+      }
+      const auto now = std::chrono::steady_clock::now();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (last == order.size()) claimed_ -= growth;  // allocated by now (or failed)
+        for (size_t k = first; k < last; ++k) {
+          const Request& r = batch[order[k]];
+          Done d;
+          d.status = status;
+          d.error = err;
+          if (status == 0) {
+            if (mbr_final_) d.mbr = std::move(mbr[k - first]);
+            else d.nodes = std::move(vit[k - first]);
+          }
+          done_[r.ticket] = std::move(d);
+          record_latency(std::chrono::duration<double, std::milli>(now - r.submitted).count());
+        }
+        outstanding_ -= n;
+        ++batches_;
+        batched_requests_ += n;
+      }
+      cv_done_.notify_all();
+      first = last;
     }
-    cv_done_.notify_all();
   }
 }
 

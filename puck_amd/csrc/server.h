@@ -8,7 +8,8 @@
 // needs (Parser::bytes_needed) fits into what the slot already holds plus the device's free memory
 // (hipMemGetInfo, less what other slots have claimed but not yet allocated and a reserve), and the
 // whole server stays within its memory budget (WorkSpace.allocate's rule, WorkSpace.scala:88-122,
-// applied per admission instead of once per call).
+// applied per admission instead of once per call). A slot parses its batch in two length classes,
+// short first, publishing each class's results as soon as it is parsed.
 #pragma once
 
 #include <atomic>
@@ -87,6 +88,8 @@ class Server {
   int64_t reserve_ = (int64_t)256 << 20;
   int64_t claimed_ = 0;                 // growth admitted by slots but not yet allocated
   int32_t device_ = 0;
+  // Length class boundary of a slot's batch (words): the short class is parsed and published first.
+  static constexpr int32_t kSplitLength = 40;
   bool fits(int32_t slot, const std::vector<int32_t>& lengths, int64_t* growth);
   void record_latency(double ms);
 };

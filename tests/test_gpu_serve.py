@@ -47,6 +47,24 @@ def test_server_coarse_to_fine(puck, wsj0, wsj1):
         assert np.array_equal(srv.result(tickets[i], len(s), wait=False), want[i])
 
 
+def test_server_length_classes_short_first(puck, wsj0, wsj1):
+    """A slot parses its batch's sentences of at most 40 words, then the longer ones: every result is
+    still the one the sentence gets alone, in Viterbi and in MBR."""
+    words, offsets = wsj1.sample(seed=79, n=48, mode=2, min_len=1, max_len=70)
+    lens = np.diff(offsets)
+    assert (lens <= 40).any() and (lens > 40).any()
+    sents = split_sentences(words, offsets)
+    for layers in ([wsj0, wsj1], [wsj0, puck.Grammar.load_text(puck.grammar_path("wsj_1"), puck.REAL)]):
+        want = puck.CLParser(layers, mem_bytes=2 << 30).parse((words, offsets))
+        srv = puck.Server(layers, mem_bytes=4 << 30, max_inflight=1, max_batch=1000)
+        tickets = [srv.submit(s) for s in sents]
+        srv.flush()
+        for i, s in enumerate(sents):
+            np.testing.assert_array_equal(srv.result(tickets[i], len(s), wait=False), want[i], err_msg=f"request {i}")
+        st = srv.stats()
+        assert st["completed"] == len(sents) and st["batches"] >= 2
+
+
 def test_server_rejects_bad_requests_at_submit(puck, wsj1):
     """A request with an unknown word id or a bad length is refused by submit, so it cannot fail the
     batch it would have joined; the requests around it complete normally."""
