@@ -965,27 +965,45 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
           my_lo = my_k < kend ? pre[my_k] - base : hi0;
           my_hi = my_k < kend ? pre[my_k + 1] - base : hi0;
         }
-        for (; k < kr; ++k) {
-          const float* p = part + (size_t)__builtin_amdgcn_readlane(ls, k) * pitch;
-          for (int pb = lo0; pb < hi0; pb += CAP) {
-            const int len = min(CAP, hi0 - pb);
+        // The run's (entry, piece) steps in order: piece q of entry e covers [lo0 + q CAP, ...) of
+        // that entry's partial row. A piece's values are loaded into registers one step ahead, so
+        // its global loads are in flight while the previous piece is folded from LDS.
+        const int np = (hi0 - lo0 + CAP - 1) / CAP;
+        const int steps = (kr - k) * np;
+        float nxt[CAP / 64];
+        auto load_step = [&](int t) {
+          const int e = k + t / np;
+          const int pb = lo0 + (t % np) * CAP;
+          const int len = min(CAP, hi0 - pb);
+          const float* p = part + (size_t)__builtin_amdgcn_readlane(ls, e) * pitch;
 #pragma unroll
-            for (int u = 0; u < CAP / 64; ++u) {
-              const int i = lane + 64 * u;
-              if (i < len) buf[i] = ld_f<COH>(p + pb + i);
-            }
-            // LDS operations of one wave complete in order; the fences keep the compiler from
-            // moving the reads above the writes (or the next piece's writes above these reads).
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int jz = use ? min(my_hi, pb + len) : 0;
-            for (int j = max(my_lo, pb); j < jz; ++j) acc = semi_add<SEMI>(acc, buf[j - pb]);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          for (int u = 0; u < CAP / 64; ++u) {
+            const int i = lane + 64 * u;
+            nxt[u] = i < len ? ld_f<COH>(p + pb + i) : 0.0f;
           }
+        };
+        if (steps > 0) load_step(0);
+        for (int t = 0; t < steps; ++t) {
+          const int pb = lo0 + (t % np) * CAP;
+          const int len = min(CAP, hi0 - pb);
+#pragma unroll
+          for (int u = 0; u < CAP / 64; ++u) {
+            const int i = lane + 64 * u;
+            if (i < len) buf[i] = nxt[u];
+          }
+          // LDS operations of one wave complete in order; the fences keep the compiler from
+          // moving the reads above the writes (or the next piece's writes above these reads).
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (t + 1 < steps) load_step(t + 1);
+          const int jz = use ? min(my_hi, pb + len) : 0;
+          for (int j = max(my_lo, pb); j < jz; ++j) acc = semi_add<SEMI>(acc, buf[j - pb]);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+        k = kr;
       }
     }
   }
