@@ -929,21 +929,29 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
           const int uc = a.uniform_count[f];
           const int total = (kr - k) * uc;
           const bool live = my_k < kend && use;
-          int ent = k, jj = 0;  // (entry, split row) of value m
-          for (int m = 0; m < total; m += kReduceBatch) {
-            float v[kReduceBatch];
+          int ent = k, jj = 0;  // (entry, split row) of the next value to load
+          // Batches of kReduceBatch values, loaded one batch ahead of the fold.
+          auto load_batch = [&](int m, float* dst) {
 #pragma unroll
             for (int u = 0; u < kReduceBatch; ++u) {
-              v[u] = semi_zero<SEMI>();
+              dst[u] = semi_zero<SEMI>();
               if (m + u < total) {
                 const int sl = __builtin_amdgcn_readlane(ls, ent);
-                if (live) v[u] = ld_f<COH>(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane);
+                if (live) dst[u] = ld_f<COH>(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane);
                 if (++jj == uc) {
                   jj = 0;
                   ++ent;
                 }
               }
             }
+          };
+          float nxt[kReduceBatch];
+          if (total > 0) load_batch(0, nxt);
+          for (int m = 0; m < total; m += kReduceBatch) {
+            float v[kReduceBatch];
+#pragma unroll
+            for (int u = 0; u < kReduceBatch; ++u) v[u] = nxt[u];
+            if (m + kReduceBatch < total) load_batch(m + kReduceBatch, nxt);
             if (SEMI == kViterbi) {
 #pragma unroll
               for (int u = 0; u < kReduceBatch; u += 2) acc = fmaxf(acc, fmaxf(v[u], v[u + 1]));
