@@ -719,12 +719,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanBatchArgs p) {
   }
 }
 
-// Stable sort of a job's cells by key (one workgroup per job): two counting-sort passes over 4-bit
-// digits, low digit first. Pass one writes (key << 24 | cell) in low-digit order into posof, pass
-// two scatters cells by the high digit into perm, then posof is rebuilt as perm's inverse.
+// Stable sort of a job's cells by key (one workgroup per job): counting-sort passes over the key's
+// 4-bit digits, low digit first (an LSD radix sort). Passes carry cell indices, alternating between
+// posof (even passes write it) and perm (odd passes), and read each cell's digit from keys; the
+// last pass writes perm, then posof is rebuilt as perm's inverse.
 __global__ __launch_bounds__(512) void k_plan_sort(PlanBatchArgs p) {
   constexpr int T = 512, D = 16;
-  static_assert(kPlanKeyBlocks == 8, "two 4-bit digits");
+  constexpr int kPasses = kPlanKeyBlocks / 4;
+  static_assert(kPlanKeyBlocks % 8 == 0, "an even number of 4-bit digits: the last pass writes perm");
   __shared__ int32_t s_a[D * T];
   __shared__ int32_t s_sum[T];
   const PlanJob& J = p.jobs[blockIdx.x];
@@ -735,12 +737,13 @@ __global__ __launch_bounds__(512) void k_plan_sort(PlanBatchArgs p) {
   const int per = (nc + T - 1) / T;
   const int lo = min(t * per, nc), hi = min(lo + per, nc);
   int32_t* tmp = v.posof;
-  for (int pass = 0; pass < 2; ++pass) {
-    auto digit = [&](int i) -> int {
-      return pass == 0 ? (v.keys[i] & (D - 1)) : ((tmp[i] >> 28) & (D - 1));
-    };
+  for (int pass = 0; pass < kPasses; ++pass) {
+    const int32_t* src = pass == 0 ? nullptr : ((pass & 1) ? tmp : v.perm);
+    int32_t* dst = (pass & 1) ? v.perm : tmp;
+    auto cell_at = [&](int i) -> int { return src != nullptr ? src[i] : i; };
+    auto digit = [&](int c) -> int { return (v.keys[c] >> (4 * pass)) & (D - 1); };
     for (int b = 0; b < D; ++b) s_a[b * T + t] = 0;
-    for (int i = lo; i < hi; ++i) ++s_a[digit(i) * T + t];
+    for (int i = lo; i < hi; ++i) ++s_a[digit(cell_at(i)) * T + t];
     __syncthreads();
     // Exclusive scan of the digit-major table: entry (d, t) becomes the sorted position of thread
     // t's first cell with digit d.
@@ -761,10 +764,9 @@ __global__ __launch_bounds__(512) void k_plan_sort(PlanBatchArgs p) {
       run += x;
     }
     __syncthreads();
-    if (pass == 0) {
-      for (int k = lo; k < hi; ++k) tmp[s_a[digit(k) * T + t]++] = (v.keys[k] << 24) | k;
-    } else {
-      for (int i = lo; i < hi; ++i) v.perm[s_a[digit(i) * T + t]++] = tmp[i] & 0xffffff;
+    for (int i = lo; i < hi; ++i) {
+      const int c = cell_at(i);
+      dst[s_a[digit(c) * T + t]++] = c;
     }
     __syncthreads();
   }

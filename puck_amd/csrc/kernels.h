@@ -154,7 +154,7 @@ struct PassArgs {
 // items share their active blocks and the rule kernel's walk over the union of its items' blocks
 // wastes little. Every job of a layer's masked passes is planned by the same four launches (count,
 // sort, scan, fill) on the parser's stream, the jobs spread over the grids.
-constexpr int kPlanKeyBlocks = 8;
+constexpr int kPlanKeyBlocks = 16;
 constexpr int kPlanThreads = 256;  // threads per block of the count and fill launches
 // Per-family data of a layer that every job of the family shares.
 struct PlanFamStatic {
