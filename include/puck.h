@@ -139,13 +139,20 @@ enum puck_parser_flags {
   PUCK_FLAG_OUTSIDE_FINAL = 16,    /* puck_parse_viterbi also runs the final layer's outside pass before
                                       the decode, as CLParser.insideOutside does
                                       (CLParser.scala:171-182); the trees do not change */
-  PUCK_FLAG_DEVICE_SCHEDULE = 32   /* each pass runs as one persistent launch that pulls the rule and
+  PUCK_FLAG_DEVICE_SCHEDULE = 32,  /* each pass runs as one persistent launch that pulls the rule and
                                       reduce work of all its spans from a device queue, a span of a
                                       sentence starting once that sentence's shorter spans are done
                                       (per-block completion counters); would replace the span loop of
                                       CLParser.scala:284-291,331-342 on the device. DISABLED: rejected
                                       with PUCK_ERR_ARG (the launch did not finish on the GPU and no
                                       cause has been shown) */
+  PUCK_FLAG_SLOT_PARTIALS = 64     /* Viterbi layers: the rule kernels write one partial row per
+                                      (operand chunk, target) slot with plain stores, as Real and
+                                      LogSum layers always do, instead of folding every slot of a
+                                      target into one row with an integer atomic max (the default; the
+                                      reference's Viterbi kernels accumulate with atomic_min on the
+                                      score bits, SimpleGenRuleMultiply.java:326-350). Results are
+                                      identical; the flag exists for comparison */
 };
 /* new CLParser(data, maxAllocSize) (CLParser.scala:46-48): mem_bytes bounds the device workspace,
  * as the reference's --mem option bounds WorkSpace.allocate (WorkSpace.scala:91-120). */
@@ -216,6 +223,7 @@ typedef struct {
   int32_t outside_final;  /* Viterbi final layer: outside pass before the decode (insideOutside) */
   int32_t device_schedule; /* must be 0: see PUCK_FLAG_DEVICE_SCHEDULE (disabled) */
   int32_t reduce_targets; /* see puck_launch_config_t */
+  int32_t slot_partials;  /* see PUCK_FLAG_SLOT_PARTIALS (0: fold, the default) */
 } puck_batch_config_t;
 typedef struct {
   double device_ms; /* first kernel of the first bucket to the last kernel of the last bucket */

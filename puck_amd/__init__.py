@@ -66,13 +66,14 @@ FLAG_GRAPHS = 4
 FLAG_NO_PLAN_AHEAD = 8  # accepted and ignored: every parser works on one HIP stream
 FLAG_OUTSIDE_FINAL = 16
 FLAG_DEVICE_SCHEDULE = 32
+FLAG_SLOT_PARTIALS = 64
 
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("streams", ctypes.c_int32), ("bucket_cells", ctypes.c_int64), ("persistent", ctypes.c_int32),
                 ("graphs", ctypes.c_int32), ("piece_target", ctypes.c_int32), ("items_per_lane", ctypes.c_int32),
                 ("outside_final", ctypes.c_int32), ("device_schedule", ctypes.c_int32),
-                ("reduce_targets", ctypes.c_int32)]
+                ("reduce_targets", ctypes.c_int32), ("slot_partials", ctypes.c_int32)]
 
 
 class BatchStats(ctypes.Structure):
@@ -319,16 +320,18 @@ class CLParser:
     """Coarse-to-fine CKY parser over grammar layers on one GPU (CLParser.scala:46-952)."""
 
     def __init__(self, layers, device=0, mem_bytes=16 << 30, items_per_lane=2, persistent=False, graphs=False,
-                 outside_final=False, device_schedule=False):
+                 outside_final=False, device_schedule=False, slot_partials=False):
         """outside_final: parse() also runs a Viterbi final layer's outside pass before the decode,
         as CLParser.insideOutside does (the trees are unchanged). device_schedule: each pass runs as
-        one persistent launch over a device queue of all its spans (PUCK_FLAG_DEVICE_SCHEDULE)."""
+        one persistent launch over a device queue of all its spans (PUCK_FLAG_DEVICE_SCHEDULE).
+        slot_partials: Viterbi layers write one partial row per rule slot instead of folding them
+        per target (PUCK_FLAG_SLOT_PARTIALS; same results)."""
         self.layers = list(layers)
         arr = (_P * len(self.layers))(*[g._h for g in self.layers])
         h = _P()
         flags = ((FLAG_ONE_ITEM_PER_LANE if items_per_lane == 1 else 0) | (FLAG_PERSISTENT if persistent else 0)
                  | (FLAG_GRAPHS if graphs else 0) | (FLAG_OUTSIDE_FINAL if outside_final else 0)
-                 | (FLAG_DEVICE_SCHEDULE if device_schedule else 0))
+                 | (FLAG_DEVICE_SCHEDULE if device_schedule else 0) | (FLAG_SLOT_PARTIALS if slot_partials else 0))
         _check(lib().puck_parser_create(arr, len(self.layers), device, mem_bytes, flags, ctypes.byref(h)))
         self._h = h
 
@@ -420,12 +423,12 @@ class BatchParser:
 
     def __init__(self, layers, device=0, mem_bytes=16 << 30, streams=3, bucket_cells=1 << 20, persistent=True,
                  graphs=True, piece_target=512, items_per_lane=2, outside_final=False, device_schedule=False,
-                 reduce_targets=4):
+                 reduce_targets=4, slot_partials=False):
         self.layers = list(layers)
         arr = (_P * len(self.layers))(*[g._h for g in self.layers])
         self.config = BatchConfig(int(streams), int(bucket_cells), int(bool(persistent)), int(bool(graphs)),
                                   int(piece_target), int(items_per_lane), int(bool(outside_final)),
-                                  int(bool(device_schedule)), int(reduce_targets))
+                                  int(bool(device_schedule)), int(reduce_targets), int(bool(slot_partials)))
         h = _P()
         _check(lib().puck_batch_create(arr, len(self.layers), device, mem_bytes, ctypes.byref(self.config),
                                        ctypes.byref(h)))

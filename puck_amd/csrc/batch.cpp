@@ -32,6 +32,7 @@ BatchParser::BatchParser(std::vector<std::shared_ptr<Grammar>> layers, int32_t d
     p.set_outside_final(cfg_.outside_final);
     p.set_device_schedule(cfg_.device_schedule);
     p.set_reduce_targets(cfg_.reduce_targets);
+    p.set_fold(cfg_.fold);
     p.use_graphs = cfg_.graphs;
   }
   hip_ok(hipSetDevice(device_), "hipSetDevice");

@@ -30,6 +30,7 @@ struct BatchConfig {
   bool outside_final = false;     // Viterbi final layer: outside pass before the decode (insideOutside)
   bool device_schedule = false;   // one persistent launch per pass with device dependency counters
   int32_t reduce_targets = 4;     // chart targets (waves) per reduce workgroup: 1, 2, 4, 8 or 16
+  bool fold = true;               // Viterbi layers fold partial rows per target (Parser::set_fold)
 };
 
 struct BatchStats {

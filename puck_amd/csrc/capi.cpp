@@ -301,6 +301,7 @@ int32_t puck_parser_create(const puck_grammar_t* const* layers, int32_t n_layers
       h->p->use_graphs = (flags & PUCK_FLAG_GRAPHS) != 0;
       h->p->set_outside_final((flags & PUCK_FLAG_OUTSIDE_FINAL) != 0);
       h->p->set_device_schedule((flags & PUCK_FLAG_DEVICE_SCHEDULE) != 0);
+      h->p->set_fold((flags & PUCK_FLAG_SLOT_PARTIALS) == 0);
     } catch (...) {
       delete h;
       throw;
@@ -570,6 +571,7 @@ int32_t puck_batch_create(const puck_grammar_t* const* layers, int32_t n_layers,
     c.outside_final = cfg->outside_final != 0;
     c.device_schedule = cfg->device_schedule != 0;
     c.reduce_targets = rt;
+    c.fold = cfg->slot_partials == 0;
     auto* h = new puck_batch;
     try {
       h->b.reset(new puck::BatchParser(gs, device, mem_bytes > 0 ? mem_bytes : (int64_t)16 << 30, c));

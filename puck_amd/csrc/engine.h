@@ -96,6 +96,10 @@ struct DevReduce {
   int32_t* d_off = nullptr;   // n_t + 1
   int32_t* d_fam = nullptr;   // index into families
   int32_t* d_slot = nullptr;  // slot within that family
+  // The same for folded partial rows: one entry per (family, target), its row = the target.
+  int32_t* d_off_f = nullptr;
+  int32_t* d_fam_f = nullptr;
+  int32_t* d_slot_f = nullptr;
 };
 
 enum ReduceId : int32_t { kRInBin = 0, kRInBin2, kRInUN, kRInUT, kROutBin, kROutUN, kROutW1, kNumReduce };
@@ -266,6 +270,20 @@ class Parser {
   int32_t piece_target_ = 512;
   int32_t queue_wgs_per_cu_ = 1;
   int32_t reduce_targets_ = 4;
+  // Folded partial rows (Viterbi semiring, on by default): every slot of the rule kernels folds its
+  // values into one partial row per target symbol with an integer atomic max on order-preserving
+  // encodings (exact and independent of order), so the reduce reads one row per (family, target)
+  // instead of one per (operand chunk, target) slot. Off: one row per slot, written with plain
+  // stores. Results are identical either way.
+  void set_fold(bool on) {
+    if (fold_ == on) return;
+    fold_ = on;
+    drop_graphs();
+  }
+  bool fold_ = true;
+  // Whether the rule launches of layer L fold (Viterbi layers with fold_ set; never in the
+  // device-scheduled pass, whose units exchange partial rows with coherent plain accesses).
+  bool folds(const Layer& L) const { return fold_ && !device_schedule && L.g.semiring == kViterbi; }
   // Graph mode of parse(): the device work of a batch is captured into a HIP graph the second
   // time a batch of the same sentence lengths (in order) is seen, and replayed after that.
   bool use_graphs = false;

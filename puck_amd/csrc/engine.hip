@@ -132,20 +132,43 @@ __device__ inline float semi_add(float acc, float v) {
   else return acc + v;
 }
 
+// Order-preserving 32-bit encoding of a float: for any two non-NaN floats x < y (with -0 < +0),
+// enc_ordered(x) < enc_ordered(y) as signed integers. Negative floats have their magnitude bits
+// flipped, so the encoding is an involution (decode = encode). Folding partial values with an
+// integer atomic max on encodings gives exactly the IEEE maximum, in any order.
+__device__ __forceinline__ int32_t enc_ordered(float v) {
+  const int32_t b = __float_as_int(v);
+  return b ^ ((b >> 31) & 0x7fffffff);
+}
+// Decodes when `mask` is 0x7fffffff and passes the bits through when it is 0.
+__device__ __forceinline__ float dec_ordered(float x, int32_t mask) {
+  const int32_t b = __float_as_int(x);
+  return __int_as_float(b ^ ((b >> 31) & mask));
+}
+typedef __attribute__((address_space(1))) int32_t GlobalI32;
+__device__ __forceinline__ void fold_max(float* p, float v) {
+  // A global (not flat) atomic: a flat one would also count on the LDS wait counter that the rule
+  // loop drains every step.
+  __hip_atomic_fetch_max((GlobalI32*)p, enc_ordered(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Encoding of the Viterbi zero (-inf): the fill value of folded partial rows.
+constexpr uint32_t kEncNegInf = 0xff800000u ^ 0x7fffffffu;
+
 // Loads and stores of data that another workgroup of the same launch produces or consumes (the
 // device-scheduled pass, COH = true): relaxed agent-scope atomics, which the compiler emits as
 // global loads and stores with the sc1 bit, coherent across the XCDs' L2 caches without a
-// whole-cache writeback or invalidate. Other launches use plain accesses (COH = false).
+// whole-cache writeback or invalidate. Other launches use plain accesses (COH = false). Both go
+// through the global address space: a flat access would also count on the LDS wait counter.
 typedef __attribute__((address_space(1))) float GlobalF32;
 template <bool COH>
 __device__ __forceinline__ float ld_f(const float* p) {
   if constexpr (COH) return __hip_atomic_load((GlobalF32*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else return *p;
+  else return *(const GlobalF32*)p;
 }
 template <bool COH>
 __device__ __forceinline__ void st_f(float* p, float v) {
   if constexpr (COH) __hip_atomic_store((GlobalF32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  else *p = v;
+  else *(GlobalF32*)p = v;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -172,9 +195,10 @@ struct RulesLds {
 // `piece` of every wave's rule stream. Every thread of the workgroup calls it with the same
 // arguments; it contains workgroup barriers before the rule loop only. COH: operand rows are read
 // and partial rows written with agent-coherent accesses (see ld_f).
-template <int IPL, int SEMI, bool UNARY, bool SKIP, bool COH = false>
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool COH = false, bool FOLD = false>
 __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk, int piece, float* smem,
                                            RulesLds<64 * IPL>& sh) {
+  static_assert(!FOLD || SEMI == kViterbi, "only max-plus partials fold by atomic max");
   constexpr int ITEMS = 64 * IPL;
   int32_t* s_cell_a = sh.cell_a;
   int32_t* s_cell_b = sh.cell_b;
@@ -355,10 +379,10 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // group of eight rules for the whole wave).
   typedef uint32_t W16 __attribute__((ext_vector_type(16)));
   typedef uint32_t W8 __attribute__((ext_vector_type(8)));
-  typedef uint32_t W2 __attribute__((ext_vector_type(2)));
+  typedef uint32_t W4 __attribute__((ext_vector_type(4)));
   typedef const __attribute__((address_space(4))) W16 ConstW16;
   typedef const __attribute__((address_space(4))) W8 ConstW8;
-  typedef const __attribute__((address_space(4))) W2 ConstW2;
+  typedef const __attribute__((address_space(4))) W4 ConstW4;
   typedef const __attribute__((address_space(4))) char ConstByte;
   typedef const __attribute__((address_space(4))) int32_t ConstI32;
   // Groups are addressed by 32-bit byte offsets from the start of the family's stream.
@@ -366,7 +390,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   ConstByte* gbase = (ConstByte*)a.rules;
   auto group_off = [&](uint32_t o) { return *(ConstW16*)(gbase + o); };
   auto group_q = [&](uint32_t o) { return *(ConstW8*)(gbase + o + 64); };
-  auto group_slots = [&](uint32_t o) { return *(ConstW2*)(gbase + o + 96); };
+  // Skip layouts: ids of the slots ending with each half group (x, y) and their targets (z, w).
+  auto group_slots = [&](uint32_t o) { return *(ConstW4*)(gbase + o + 96); };
 
   // Block ranges of this wave's stream, visited in order. A skip layout lets a masked tile pass
   // over the blocks that none of its items allows; the cursor (cb, c_end) belongs to the furthest
@@ -413,6 +438,12 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     if (gi >= g_end) return;
     g1 = gi + GB;
   }
+  // Folded partial rows (Viterbi): each flush folds into its slot's target row. Plain streams end
+  // their slots in slot-id order, so the next slot's target is loaded one flush ahead (the table
+  // has a padding entry past the last slot); skip layouts carry the targets in their groups.
+  ConstI32* slot_tgt = (ConstI32*)a.slot_target;
+  uint32_t next_tgt = 0;
+  if constexpr (!SKIP && FOLD) next_tgt = (uint32_t)slot_tgt[slot_ctr];
 
   // Per-item scale, kept in registers for the slot flushes.
   float scl[IPL];
@@ -457,7 +488,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // Plain streams flush their slots in order from p_slot on, so the partial row advances by one
   // pitch per flush; block-major streams name each flushed slot.
   float* row = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)slot_ctr * pitch_v;
-  auto flush = [&](uint32_t slot) {
+  auto flush = [&](uint32_t slot, uint32_t tgt) {
     // Slot done: scale (Real) and write the partial. Under masks, an item whose cell does not
     // allow the slot's block also gets a value; the reduce never reads it, since it takes a
     // (cell, target) partial only where the plan's active-block bits allow the target's block.
@@ -466,6 +497,29 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     for (int i = 0; i < IPL; ++i) {
       v[i] = SEMI == kReal ? acc[i] * scl[i] : acc[i];
       acc[i] = zero;
+    }
+    if constexpr (FOLD) {
+      {
+        uint32_t t = tgt;
+        if constexpr (!SKIP) {
+          t = next_tgt;
+          next_tgt = (uint32_t)slot_tgt[++slot_ctr];
+        }
+        float* out = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)t * pitch_v;
+        if constexpr (IPL == 2 && !SKIP) {
+          if (__builtin_amdgcn_readfirstlane(pair_v)) {
+            const int m = (tile0 >> 1) + lane;
+            if (2 * m < num_items) fold_max(out + m, semi_pair<SEMI>(v[0], v[1]));
+            return;
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < IPL; ++i) {
+          const int item = tile0 + lane * IPL + i;
+          if (item < num_items) fold_max(out + item, v[i]);
+        }
+        return;
+      }
     }
     float* out;
     if constexpr (SKIP) {
@@ -487,7 +541,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       if (item < num_items) st_f<COH>(out + item, v[i]);
     }
   };
-  auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, float (*da)[IPL], float (*db)[IPL], int h) {
+  auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, uint32_t tgt, float (*da)[IPL], float (*db)[IPL],
+                  int h) {
 #pragma unroll
     for (int u = h * H; u < h * H + H; ++u) {
       const float q = __uint_as_float(qs[u]);
@@ -505,7 +560,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       }
     }
     if (tag & kSlotEnd) {
-      flush(slot);
+      flush(slot, tgt);
     }
   };
   // Software pipeline over groups: while group gi is computed from (da, db) with weights cur_q
@@ -514,8 +569,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // loaded into SGPRs. Scalar loads share the LDS wait counter, so the one wait the compiler
   // places before the next step's LDS reads covers both. Register sets alternate between two
   // unrolled steps, so nothing is copied.
-  auto step = [&](const W8& cur_q, const W2& cur_s, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
-                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q, W2& nxt_s,
+  auto step = [&](const W8& cur_q, const W4& cur_s, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
+                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q, W4& nxt_s,
                   float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
     // Wait for the previous step's LDS reads and scalar loads before this step's reads issue; a
     // wait the compiler places later (before the first use of da/db) would drain them as well.
@@ -528,8 +583,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     nxt_q = group_q(g1);
     if constexpr (SKIP) nxt_s = group_slots(g1);
     __builtin_amdgcn_sched_barrier(0);
-    half(cur_q, cur_t0, cur_s.x, da, db, 0);
-    half(cur_q, cur_t1, cur_s.y, da, db, 1);
+    half(cur_q, cur_t0, cur_s.x, cur_s.z, da, db, 0);
+    half(cur_q, cur_t1, cur_s.y, cur_s.w, da, db, 1);
     __builtin_amdgcn_sched_barrier(0);
     gi = g1;
     g1 = g2;
@@ -539,7 +594,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   auto done = [&]() { return SKIP ? gi == sentinel : gi >= g_end; };
   W16 oA = group_off(gi), oB = group_off(g1);
   W8 qA = group_q(gi), qB;
-  W2 sA = {0u, 0u}, sB = {0u, 0u};
+  W4 sA = {0u, 0u, 0u, 0u}, sB = {0u, 0u, 0u, 0u};
   if constexpr (SKIP) sA = group_slots(gi);
   uint32_t tA0 = oA[H - 1], tA1 = oA[G - 1], tB0 = 0, tB1 = 0;
   float a0[G][IPL], b0[G][IPL], a1[G][IPL], b1[G][IPL];
@@ -553,11 +608,11 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
 }
 
 // Plain form: one launch per rule family and span, grid (tiles, chunks, pieces).
-template <int IPL, int SEMI, bool UNARY, bool SKIP>
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool FOLD>
 __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
   extern __shared__ __align__(16) float smem[];
   __shared__ RulesLds<64 * IPL> sh;
-  rules_unit<IPL, SEMI, UNARY, SKIP>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem, sh);
+  rules_unit<IPL, SEMI, UNARY, SKIP, false, FOLD>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem, sh);
 }
 
 // Persistent form: the rule families of one span share one launch of about one workgroup per CU.
@@ -565,7 +620,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
 // workgroup pops the next unit from a device counter (zeroed before the launch) until the queue
 // is empty. Planned families take their item counts, hence their tile counts, from the device, so
 // the launch needs no host knowledge of the pruned work.
-template <int IPL, int SEMI, bool UNARY, bool SKIP>
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool FOLD>
 __global__ __launch_bounds__(kBlockThreads) void k_rules_q(QueueArgs q) {
   constexpr int ITEMS = 64 * IPL;
   extern __shared__ __align__(16) float smem[];
@@ -596,7 +651,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules_q(QueueArgs q) {
     const int cp = r / tiles;
     const int chunk = cp % q.fam[f].num_chunks;
     const int piece = cp / q.fam[f].num_chunks;
-    rules_unit<IPL, SEMI, UNARY, SKIP>(q.fam[f], tile, chunk, piece, smem, sh);
+    rules_unit<IPL, SEMI, UNARY, SKIP, false, FOLD>(q.fam[f], tile, chunk, piece, smem, sh);
   }
 }
 
@@ -829,6 +884,9 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
   const int my_c = (a.perm != nullptr && my_k < kend) ? a.perm[my_k] : my_k;  // slice cell
   float acc = semi_zero<SEMI>();
   float* buf = s_buf[w];
+  // Folded partial rows hold order-preserving encodings: decode every loaded value.
+  const int32_t dmask = a.fold ? 0x7fffffff : 0;
+  auto ldp = [&](const float* p) { return dec_ordered(ld_f<COH>(p), dmask); };
   if (t < a.n_t && a.act[0] != nullptr) {
     // Masked (planned) groups: a cell's items are contiguous in the plan's order and few, so each
     // lane reads its own items straight from the partial rows, four slots of a family at a time.
@@ -860,10 +918,10 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
             // Two items at a time: eight loads in flight, folded in (item, slot) order.
             int j = lo;
             for (; j + 2 <= hi; j += 2) {
-              const float v0 = ld_f<COH>(p0 + j), v1 = ld_f<COH>(p1 + j), v2 = ld_f<COH>(p2 + j),
-                          v3 = ld_f<COH>(p3 + j);
-              const float w0 = ld_f<COH>(p0 + j + 1), w1 = ld_f<COH>(p1 + j + 1), w2 = ld_f<COH>(p2 + j + 1),
-                          w3 = ld_f<COH>(p3 + j + 1);
+              const float v0 = ldp(p0 + j), v1 = ldp(p1 + j), v2 = ldp(p2 + j),
+                          v3 = ldp(p3 + j);
+              const float w0 = ldp(p0 + j + 1), w1 = ldp(p1 + j + 1), w2 = ldp(p2 + j + 1),
+                          w3 = ldp(p3 + j + 1);
               if (SEMI == kViterbi) {
                 acc = fmaxf(fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3))), fmaxf(fmaxf(w0, w1), fmaxf(w2, w3)));
               } else {
@@ -878,8 +936,8 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
               }
             }
             if (j < hi) {
-              const float v0 = ld_f<COH>(p0 + j), v1 = ld_f<COH>(p1 + j), v2 = ld_f<COH>(p2 + j),
-                          v3 = ld_f<COH>(p3 + j);
+              const float v0 = ldp(p0 + j), v1 = ldp(p1 + j), v2 = ldp(p2 + j),
+                          v3 = ldp(p3 + j);
               if (SEMI == kViterbi) {
                 acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
               } else {
@@ -892,7 +950,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
           }
           for (; q < kr; ++q) {
             const float* p0 = base + (size_t)__builtin_amdgcn_readlane(ls, q) * pitch;
-            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, ld_f<COH>(p0 + j));
+            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, ldp(p0 + j));
           }
         }
         k = kr;
@@ -939,7 +997,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
               dst[u] = semi_zero<SEMI>();
               if (m + u < total) {
                 const int sl = __builtin_amdgcn_readlane(ls, ent);
-                if (live) dst[u] = ld_f<COH>(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane);
+                if (live) dst[u] = ldp(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane);
                 if (++jj == uc) {
                   jj = 0;
                   ++ent;
@@ -989,7 +1047,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
 #pragma unroll
           for (int u = 0; u < CAP / 64; ++u) {
             const int i = lane + 64 * u;
-            nxt[u] = i < len ? ld_f<COH>(p + pb + i) : 0.0f;
+            nxt[u] = i < len ? ldp(p + pb + i) : 0.0f;
           }
         };
         if (steps > 0) load_step(0);
@@ -1144,6 +1202,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_pass(PassArgs p) {
 }
 
 __global__ void k_fill(float* p, size_t n, float v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Fills n dwords with a bit pattern (folded partial rows: the encoded semiring zero).
+__global__ void k_fill_bits(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
@@ -1486,6 +1550,28 @@ __global__ void k_get_root(float* out, const float* inside, const int32_t* cell_
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= num_sent) return;
   out[i] = inside[(size_t)(cell_offsets[i + 1] - 1) * cell_size + root];
+}
+
+// Every rule kernel instance of one (items per lane, semiring, fold) (plain and persistent forms, the
+// unary, block-major and plain layouts), for setting their LDS limits.
+template <int IPL, int S, bool FD>
+void rule_kernels(std::vector<const void*>& v) {
+  v.insert(v.end(), {(const void*)k_rules<IPL, S, false, false, FD>, (const void*)k_rules<IPL, S, true, false, FD>,
+                     (const void*)k_rules<IPL, S, false, true, FD>, (const void*)k_rules_q<IPL, S, false, false, FD>,
+                     (const void*)k_rules_q<IPL, S, true, false, FD>, (const void*)k_rules_q<IPL, S, false, true, FD>});
+}
+
+// Rule kernels: the semiring as above and, for the Viterbi semiring only, whether partial rows fold.
+template <class F>
+void launch_rules(int semi, bool fold, F&& f) {
+  if (semi == kViterbi) {
+    if (fold) f(std::integral_constant<int, kViterbi>{}, std::true_type{});
+    else f(std::integral_constant<int, kViterbi>{}, std::false_type{});
+  } else if (semi == kLogSum) {
+    f(std::integral_constant<int, kLogSum>{}, std::false_type{});
+  } else {
+    f(std::integral_constant<int, kReal>{}, std::false_type{});
+  }
 }
 
 template <class F>
@@ -1838,7 +1924,10 @@ void Layer::build_family(int32_t id) {
             words[u] = r.x | r.tag;
             words[kGroupRules + u] = r.y;
             words[2 * kGroupRules + u] = r.q;
-            if (r.tag && block_major) words[3 * kGroupRules + u / kRuleUnroll] = (uint32_t)r.slot;
+            if (r.tag && block_major) {
+              words[3 * kGroupRules + u / kRuleUnroll] = (uint32_t)r.slot;
+              words[3 * kGroupRules + 2 + u / kRuleUnroll] = (uint32_t)F.h_slot_target[r.slot];
+            }
           }
           packed.insert(packed.end(), words, words + gw);
         }
@@ -1855,7 +1944,12 @@ void Layer::build_family(int32_t id) {
       std::memcpy(&packed[(size_t)(F.sentinel_group + k) * gw + 2 * kGroupRules + u], &null_q, 4);
   F.num_slots = (int32_t)F.h_slot_target.size();
   F.d_chunk_range = upload(F.h_chunk_range);
-  F.d_slot_target = upload(F.h_slot_target);
+  {
+    // One padding entry: a plain stream's rule loop loads the target of the slot after its last.
+    std::vector<int32_t> st = F.h_slot_target;
+    st.push_back(0);
+    F.d_slot_target = upload(st);
+  }
   std::vector<double> block_rules(std::max(1, F.unary ? 1 : F.num_blocks), 0.0);
   for (auto& r : rules) block_rules[F.unary ? 0 : std::min(bot[r.t], (int)block_rules.size() - 1)] += 1.0;
   // The blocks with most rules key the mask plan's cell order.
@@ -1893,6 +1987,21 @@ void Layer::build_reduce(int32_t rid, std::vector<int32_t> fams, int32_t n_t) {
   R.d_off = upload(off);
   R.d_fam = upload(fv);
   R.d_slot = upload(sv);
+  // Folded rows: one entry per (family, target) with slots of that target; its row is the target.
+  std::vector<int32_t> off_f(1, 0), fv_f, sv_f;
+  for (int32_t t = 0; t < n_t; ++t) {
+    int32_t last = -1;
+    for (auto& pr : per[t])
+      if (pr.first != last) {
+        fv_f.push_back(pr.first);
+        sv_f.push_back(t);
+        last = pr.first;
+      }
+    off_f.push_back((int32_t)fv_f.size());
+  }
+  R.d_off_f = upload(off_f);
+  R.d_fam_f = upload(fv_f);
+  R.d_slot_f = upload(sv_f);
 }
 
 // ================================================================================================
@@ -2002,27 +2111,16 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   }
   // The rule kernels stage up to kLdsBudgetBytes of operand rows in dynamic LDS.
   const int lds = kLdsBudgetBytes;
-  for (const void* fn : {(const void*)k_rules<2, kViterbi, false, false>, (const void*)k_rules<2, kViterbi, true, false>,
-                         (const void*)k_rules<2, kViterbi, false, true>, (const void*)k_rules<2, kReal, false, false>,
-                         (const void*)k_rules<2, kReal, true, false>, (const void*)k_rules<2, kReal, false, true>,
-                         (const void*)k_rules<1, kViterbi, false, false>, (const void*)k_rules<1, kViterbi, true, false>,
-                         (const void*)k_rules<1, kViterbi, false, true>, (const void*)k_rules<1, kReal, false, false>,
-                         (const void*)k_rules<1, kReal, true, false>, (const void*)k_rules<1, kReal, false, true>})
-    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  for (const void* fn : {(const void*)k_rules_q<2, kViterbi, false, false>, (const void*)k_rules_q<2, kViterbi, true, false>,
-                         (const void*)k_rules_q<2, kViterbi, false, true>, (const void*)k_rules_q<2, kReal, false, false>,
-                         (const void*)k_rules_q<2, kReal, true, false>, (const void*)k_rules_q<2, kReal, false, true>,
-                         (const void*)k_rules_q<1, kViterbi, false, false>, (const void*)k_rules_q<1, kViterbi, true, false>,
-                         (const void*)k_rules_q<1, kViterbi, false, true>, (const void*)k_rules_q<1, kReal, false, false>,
-                         (const void*)k_rules_q<1, kReal, true, false>, (const void*)k_rules_q<1, kReal, false, true>})
-    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  for (const void* fn : {(const void*)k_rules<2, kLogSum, false, false>, (const void*)k_rules<2, kLogSum, true, false>,
-                         (const void*)k_rules<2, kLogSum, false, true>, (const void*)k_rules<1, kLogSum, false, false>,
-                         (const void*)k_rules<1, kLogSum, true, false>, (const void*)k_rules<1, kLogSum, false, true>,
-                         (const void*)k_rules_q<2, kLogSum, false, false>, (const void*)k_rules_q<2, kLogSum, true, false>,
-                         (const void*)k_rules_q<2, kLogSum, false, true>, (const void*)k_rules_q<1, kLogSum, false, false>,
-                         (const void*)k_rules_q<1, kLogSum, true, false>, (const void*)k_rules_q<1, kLogSum, false, true>})
-    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  std::vector<const void*> fns;
+  rule_kernels<2, kViterbi, false>(fns);
+  rule_kernels<2, kViterbi, true>(fns);
+  rule_kernels<2, kReal, false>(fns);
+  rule_kernels<2, kLogSum, false>(fns);
+  rule_kernels<1, kViterbi, false>(fns);
+  rule_kernels<1, kViterbi, true>(fns);
+  rule_kernels<1, kReal, false>(fns);
+  rule_kernels<1, kLogSum, false>(fns);
+  for (const void* fn : fns) HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   for (const void* fn : {(const void*)k_pass<2, kViterbi, false>, (const void*)k_pass<2, kViterbi, true>,
                          (const void*)k_pass<2, kReal, false>, (const void*)k_pass<2, kReal, true>,
                          (const void*)k_pass<2, kLogSum, false>, (const void*)k_pass<2, kLogSum, true>,
@@ -2236,6 +2334,8 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.rules = F.d_rules;
   a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
   a.partial = partial;
+  a.slot_target = F.d_slot_target;
+  a.fold = folds(L) ? 1 : 0;
   const int items_tile = 64 * L.ipl;
   a.pitch = (int32_t)(((a.pair_splits ? nitems / 2 : nitems) + 255) / 256 * 256);
   a.span = span;
@@ -2276,16 +2376,17 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
     t1 = next_event();
     HIP_CHECK(hipEventRecord(t0, stream_));
   }
-  launch_semi(L.g.semiring, [&](auto semi) {
+  launch_rules(L.g.semiring, a.fold != 0, [&](auto semi, auto fold) {
     constexpr int S = decltype(semi)::value;
+    constexpr bool FD = decltype(fold)::value;
     if (L.ipl == 2) {
-      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<2, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else hipLaunchKernelGGL((k_rules<2, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<2, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<2, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
     } else {
-      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<1, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else hipLaunchKernelGGL((k_rules<1, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, a);
+      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<1, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<1, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
     }
   });
   HIP_CHECK(hipGetLastError());
@@ -2345,16 +2446,17 @@ void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids
     t1 = next_event();
     HIP_CHECK(hipEventRecord(t0, stream_));
   }
-  launch_semi(L.g.semiring, [&](auto semi) {
+  launch_rules(L.g.semiring, q.fam[0].fold != 0, [&](auto semi, auto fold) {
     constexpr int S = decltype(semi)::value;
+    constexpr bool FD = decltype(fold)::value;
     if (L.ipl == 2) {
-      if (F0.unary) hipLaunchKernelGGL((k_rules_q<2, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<2, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else hipLaunchKernelGGL((k_rules_q<2, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, q);
+      if (F0.unary) hipLaunchKernelGGL((k_rules_q<2, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<2, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else hipLaunchKernelGGL((k_rules_q<2, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
     } else {
-      if (F0.unary) hipLaunchKernelGGL((k_rules_q<1, S, true, false>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<1, S, false, true>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else hipLaunchKernelGGL((k_rules_q<1, S, false, false>), grid, dim3(kBlockThreads), lds, stream_, q);
+      if (F0.unary) hipLaunchKernelGGL((k_rules_q<1, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<1, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else hipLaunchKernelGGL((k_rules_q<1, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
     }
   });
   HIP_CHECK(hipGetLastError());
@@ -2445,11 +2547,14 @@ std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t 
   std::vector<std::pair<int32_t, int32_t>> out;
   if (budget_floats < 0) budget_floats = (int64_t)(partial_.bytes / 4);
   int64_t fixed = 0;
+  // Partial rows per family: one per slot, or one per target symbol when the layer folds.
+  const bool fold = folds(L);
+  auto prow = [&](const DevFamily& F) { return (int64_t)(fold ? F.n_t : F.num_slots); };
   // Row padding: pitches round up to 256 values, block-major uniform rows to whole 64-cell blocks.
   for (int fid : R.families) {
     const DevFamily& F = L.fam[fid];
     const int64_t rows = uniform_kind(F.kind) ? (F.kind == 0 ? std::max(span - 1, 1) : 1) : 0;
-    fixed += (256 + 64 * rows) * (int64_t)F.num_slots;
+    fixed += (256 + 64 * rows) * prow(F);
   }
   int32_t k0 = 0;
   while (k0 < ncells) {
@@ -2457,7 +2562,7 @@ std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t 
     int64_t used = 0;
     while (k1 < ncells) {
       int64_t add = 0;
-      for (int fid : R.families) add += family_items(B, li, fid, span, k1, 1) * L.fam[fid].num_slots;
+      for (int fid : R.families) add += family_items(B, li, fid, span, k1, 1) * prow(L.fam[fid]);
       if (k1 > k0 && used + add + fixed > budget_floats) break;
       used += add;
       ++k1;
@@ -2897,6 +3002,7 @@ size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span,
   gs.nfam = (int32_t)nfam;
   RedArgs& ra = gs.ra;
   size_t off = 0;
+  const bool fold = folds(L);
   for (size_t fi = 0; fi < nfam; ++fi) {
     const int fid = R.families[fi];
     const DevFamily& F = L.fam[fid];
@@ -2931,7 +3037,7 @@ size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span,
     gs.valid[fi] = family_args(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? ps->pfs[fi].pre : nullptr,
                                use_masks ? ps->pfs[fi].items : nullptr, use_masks ? ps->pfs[fi].act : nullptr,
                                &gs.fa[fi], &gs.grid[fi], &gs.lds[fi]);
-    off += (size_t)pitch * F.num_slots;
+    off += (size_t)pitch * (fold ? F.n_t : F.num_slots);
   }
   float* inside = (float*)inside_.p;
   float* outside = (float*)outside_.p;
@@ -2942,9 +3048,10 @@ size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span,
   ra.cell_offsets = B.d_cell_offsets;
   ra.lengths = B.d_lengths;
   ra.cell_list = B.d_cells + B.span_start[span] + k0;
-  ra.off = R.d_off;
-  ra.fam = R.d_fam;
-  ra.slot = R.d_slot;
+  ra.fold = fold ? 1 : 0;
+  ra.off = fold ? R.d_off_f : R.d_off;
+  ra.fam = fold ? R.d_fam_f : R.d_fam;
+  ra.slot = fold ? R.d_slot_f : R.d_slot;
   ra.num_cells = kc;
   ra.cell_size = g.cell_size;
   ra.n_t = R.n_t;
@@ -2973,7 +3080,12 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       ps = &(*planned)[si];
     }
     GroupSlice gs;
-    group_slice_args(B, li, rid, span, use_masks, ps, k0, kc, (float*)partial_.p, &gs);
+    const size_t used = group_slice_args(B, li, rid, span, use_masks, ps, k0, kc, (float*)partial_.p, &gs);
+    // Folded rows start at the encoded semiring zero; every slot of a target maxes into them. (A
+    // group whose families have no items at this span uses no rows.)
+    if (gs.ra.fold && used > 0)
+      hipLaunchKernelGGL(k_fill_bits, dim3((unsigned)std::min<size_t>((used + 255) / 256, 4096)), dim3(256), 0,
+                         stream_, (uint32_t*)partial_.p, used, kEncNegInf);
     if (!persistent) {
       for (size_t fi = 0; fi < nfam; ++fi)
         launch_family(B, li, gs.fid[fi], span, use_masks, k0, kc, gs.partial[fi], use_masks ? ps->pfs[fi].pre : nullptr,

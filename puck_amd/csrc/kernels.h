@@ -17,7 +17,8 @@ constexpr int kGroupRules = 8;        // rules per pipelined group of the rule s
 // (the last rule of each half group) carry the half group's tag in bits [24, 32): kSlotEnd when
 // a slot ends there, with the slot's block bit in bits [24, 31). Streams that tiles may skip
 // through (block-major layouts) use kSkipGroupWords: the same words plus, at [24] and [25], the
-// id of the slot ending with each half group, and padding.
+// id of the slot ending with each half group, at [26] and [27] that slot's target symbol, and
+// padding.
 constexpr int kGroupWords = 24;
 constexpr int kSkipGroupWords = 32;
 constexpr uint32_t kOffsetMask = (1u << 18) - 1;
@@ -51,7 +52,12 @@ struct FamArgs {
   const int32_t* wave_blocks;
   const uint32_t* rules;  // rule groups, kGroupWords dwords each
   const int32_t* block_of_target;
+  // Partial rows: one per slot (row = slot id, plain stores), or with `fold` one per target symbol
+  // (row = the slot's target), into which every slot of that target folds its values by an
+  // integer atomic max on order-preserving encodings (see enc_ordered); Viterbi semiring only.
   float* partial;
+  const int32_t* slot_target;  // target symbol of each slot (one padding entry past the last)
+  int32_t fold;
   int32_t num_cells, num_items, uniform_count, slice_cell0;
   int32_t num_block_ranges, sentinel_group;
   int32_t pieces_log2;  // the launch cuts streams into 2^pieces_log2 pieces (grid z)
@@ -79,7 +85,8 @@ struct RedArgs {
   const int32_t* cell_list;
   const int32_t* off;
   const int32_t* fam;
-  const int32_t* slot;
+  const int32_t* slot;  // partial row of each entry: a slot id, or with `fold` the target itself
+  int32_t fold;         // partial values are order-preserving encodings (see FamArgs::fold)
   const float* partial[kMaxGroupFamilies];
   const int32_t* prefix[kMaxGroupFamilies];  // slice item prefix, null = uniform count
   int32_t uniform_count[kMaxGroupFamilies];
