@@ -67,6 +67,9 @@ struct DevFamily {
   int32_t t_half = 0, a_half = 0, b_half = 0;  // ChartHalf of target and operands
   int32_t n_t = 0, n_a = 0, n_b = 0;           // symbol space sizes
   bool unary = false;
+  // Binary families fold their partial rows per target when the layer folds (unary families, with
+  // only a few operand chunks per target, keep one row per slot).
+  bool foldable = false;
   int32_t num_chunks = 0, num_slots = 0, num_rules = 0, max_rows = 0;
   std::vector<int32_t> h_chunk_range;       // 4 per chunk: a_lo, a_hi, b_lo, b_hi
   std::vector<int32_t> h_chunk_slot_start;  // num_chunks + 1
@@ -96,7 +99,8 @@ struct DevReduce {
   int32_t* d_off = nullptr;   // n_t + 1
   int32_t* d_fam = nullptr;   // index into families
   int32_t* d_slot = nullptr;  // slot within that family
-  // The same for folded partial rows: one entry per (family, target), its row = the target.
+  // The same when the layer folds: a foldable family has one entry per target it feeds (row = the
+  // target), the others one per slot as above.
   int32_t* d_off_f = nullptr;
   int32_t* d_fam_f = nullptr;
   int32_t* d_slot_f = nullptr;
@@ -393,6 +397,7 @@ class Parser {
     size_t lds[kMaxGroupFamilies] = {};
     float* partial[kMaxGroupFamilies] = {};
     int32_t apps_slot[kMaxGroupFamilies] = {};
+    int64_t row_items[kMaxGroupFamilies] = {};  // values per partial row (unpruned upper bound)
     RedArgs ra = {};
     dim3 rgrid;
   };

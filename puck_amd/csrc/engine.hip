@@ -884,9 +884,9 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
   const int my_c = (a.perm != nullptr && my_k < kend) ? a.perm[my_k] : my_k;  // slice cell
   float acc = semi_zero<SEMI>();
   float* buf = s_buf[w];
-  // Folded partial rows hold order-preserving encodings: decode every loaded value.
-  const int32_t dmask = a.fold ? 0x7fffffff : 0;
-  auto ldp = [&](const float* p) { return dec_ordered(ld_f<COH>(p), dmask); };
+  // Folded partial rows hold order-preserving encodings: every loaded value of family f is
+  // decoded with that family's mask.
+  auto ldp = [&](const float* p, int32_t dm) { return dec_ordered(ld_f<COH>(p), dm); };
   if (t < a.n_t && a.act[0] != nullptr) {
     // Masked (planned) groups: a cell's items are contiguous in the plan's order and few, so each
     // lane reads its own items straight from the partial rows, four slots of a family at a time.
@@ -909,6 +909,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
           const int lo = pre[pos], hi = pre[pos + 1];
           const size_t pitch = (size_t)a.pitch[f];
           const float* base = a.partial[f];
+          const int32_t dm = a.dmask[f];
           int q = k;
           for (; q + 4 <= kr; q += 4) {
             const float* p0 = base + (size_t)__builtin_amdgcn_readlane(ls, q) * pitch;
@@ -918,10 +919,10 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
             // Two items at a time: eight loads in flight, folded in (item, slot) order.
             int j = lo;
             for (; j + 2 <= hi; j += 2) {
-              const float v0 = ldp(p0 + j), v1 = ldp(p1 + j), v2 = ldp(p2 + j),
-                          v3 = ldp(p3 + j);
-              const float w0 = ldp(p0 + j + 1), w1 = ldp(p1 + j + 1), w2 = ldp(p2 + j + 1),
-                          w3 = ldp(p3 + j + 1);
+              const float v0 = ldp(p0 + j, dm), v1 = ldp(p1 + j, dm), v2 = ldp(p2 + j, dm),
+                          v3 = ldp(p3 + j, dm);
+              const float w0 = ldp(p0 + j + 1, dm), w1 = ldp(p1 + j + 1, dm), w2 = ldp(p2 + j + 1, dm),
+                          w3 = ldp(p3 + j + 1, dm);
               if (SEMI == kViterbi) {
                 acc = fmaxf(fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3))), fmaxf(fmaxf(w0, w1), fmaxf(w2, w3)));
               } else {
@@ -936,8 +937,8 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
               }
             }
             if (j < hi) {
-              const float v0 = ldp(p0 + j), v1 = ldp(p1 + j), v2 = ldp(p2 + j),
-                          v3 = ldp(p3 + j);
+              const float v0 = ldp(p0 + j, dm), v1 = ldp(p1 + j, dm), v2 = ldp(p2 + j, dm),
+                          v3 = ldp(p3 + j, dm);
               if (SEMI == kViterbi) {
                 acc = fmaxf(acc, fmaxf(fmaxf(v0, v1), fmaxf(v2, v3)));
               } else {
@@ -950,7 +951,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
           }
           for (; q < kr; ++q) {
             const float* p0 = base + (size_t)__builtin_amdgcn_readlane(ls, q) * pitch;
-            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, ldp(p0 + j));
+            for (int j = lo; j < hi; ++j) acc = semi_add<SEMI>(acc, ldp(p0 + j, dm));
           }
         }
         k = kr;
@@ -982,6 +983,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
           }
         }
         const float* part = a.partial[f];
+        const int32_t dm = a.dmask[f];
         const size_t pitch = (size_t)a.pitch[f];
         if (a.prefix[f] == nullptr) {
           // Block-major items: split row j of this block's 64 cells is item (block * uc + j) * 64 +
@@ -997,7 +999,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
               dst[u] = semi_zero<SEMI>();
               if (m + u < total) {
                 const int sl = __builtin_amdgcn_readlane(ls, ent);
-                if (live) dst[u] = ldp(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane);
+                if (live) dst[u] = ldp(part + (size_t)sl * pitch + ((size_t)bx * uc + jj) * 64 + lane, dm);
                 if (++jj == uc) {
                   jj = 0;
                   ++ent;
@@ -1047,7 +1049,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
 #pragma unroll
           for (int u = 0; u < CAP / 64; ++u) {
             const int i = lane + 64 * u;
-            nxt[u] = i < len ? ldp(p + pb + i) : 0.0f;
+            nxt[u] = i < len ? ldp(p + pb + i, dm) : 0.0f;
           }
         };
         if (steps > 0) load_step(0);
@@ -1206,10 +1208,16 @@ __global__ void k_fill(float* p, size_t n, float v) {
     p[i] = v;
 }
 
-// Fills n dwords with a bit pattern (folded partial rows: the encoded semiring zero).
-__global__ void k_fill_bits(uint32_t* p, size_t n, uint32_t v) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    p[i] = v;
+// Folded partial rows of a slice start at the encoded semiring zero (FoldFillArgs): family
+// blockIdx.y, rows strided over blockIdx.x, values over the threads.
+__global__ __launch_bounds__(256) void k_fill_fold(FoldFillArgs f) {
+  const int fi = blockIdx.y;
+  const int len = f.pre[fi] != nullptr ? f.pre[fi][f.num_cells[fi]] : f.len[fi];
+  uint32_t* base = reinterpret_cast<uint32_t*>(f.base[fi]);
+  for (int r = blockIdx.x; r < f.rows[fi]; r += gridDim.x) {
+    uint32_t* row = base + (size_t)r * f.pitch[fi];
+    for (int i = threadIdx.x; i < len; i += blockDim.x) row[i] = kEncNegInf;
+  }
 }
 
 // Span-1 tag scores (CLParser.scala:411-467): one thread per word position.
@@ -1735,6 +1743,7 @@ void Layer::build_family(int32_t id) {
   }
   F.num_rules = (int32_t)rules.size();
   F.rule_apps_per_item = (double)rules.size();
+  F.foldable = !F.unary;
 
   // Chunking of the operand symbol spaces so that one tile's rows fit the LDS budget (one
   // workgroup per CU with two items per lane, two workgroups per CU with one).
@@ -1987,16 +1996,17 @@ void Layer::build_reduce(int32_t rid, std::vector<int32_t> fams, int32_t n_t) {
   R.d_off = upload(off);
   R.d_fam = upload(fv);
   R.d_slot = upload(sv);
-  // Folded rows: one entry per (family, target) with slots of that target; its row is the target.
+  // Folding layers: a foldable family's slots of one target share one row, the target's.
   std::vector<int32_t> off_f(1, 0), fv_f, sv_f;
   for (int32_t t = 0; t < n_t; ++t) {
     int32_t last = -1;
-    for (auto& pr : per[t])
-      if (pr.first != last) {
-        fv_f.push_back(pr.first);
-        sv_f.push_back(t);
-        last = pr.first;
-      }
+    for (auto& pr : per[t]) {
+      const bool f = fam[fams[pr.first]].foldable;
+      if (f && pr.first == last) continue;
+      fv_f.push_back(pr.first);
+      sv_f.push_back(f ? t : pr.second);
+      last = pr.first;
+    }
     off_f.push_back((int32_t)fv_f.size());
   }
   R.d_off_f = upload(off_f);
@@ -2335,7 +2345,7 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
   a.partial = partial;
   a.slot_target = F.d_slot_target;
-  a.fold = folds(L) ? 1 : 0;
+  a.fold = folds(L) && F.foldable ? 1 : 0;
   const int items_tile = 64 * L.ipl;
   a.pitch = (int32_t)(((a.pair_splits ? nitems / 2 : nitems) + 255) / 256 * 256);
   a.span = span;
@@ -2549,7 +2559,7 @@ std::vector<std::pair<int32_t, int32_t>> Parser::group_slices(Batch& B, int32_t 
   int64_t fixed = 0;
   // Partial rows per family: one per slot, or one per target symbol when the layer folds.
   const bool fold = folds(L);
-  auto prow = [&](const DevFamily& F) { return (int64_t)(fold ? F.n_t : F.num_slots); };
+  auto prow = [&](const DevFamily& F) { return (int64_t)(fold && F.foldable ? F.n_t : F.num_slots); };
   // Row padding: pitches round up to 256 values, block-major uniform rows to whole 64-cell blocks.
   for (int fid : R.families) {
     const DevFamily& F = L.fam[fid];
@@ -3029,15 +3039,18 @@ size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span,
       row_items = uniform_items(kc, span / 2);
     }
     const int32_t pitch = (int32_t)((row_items + 255) / 256 * 256);
+    const bool ffold = fold && F.foldable;
     ra.partial[fi] = base + off;
     ra.pitch[fi] = pitch;
+    ra.dmask[fi] = ffold ? 0x7fffffff : 0;
+    gs.row_items[fi] = row_items;
     gs.fid[fi] = fid;
     gs.partial[fi] = base + off;
     gs.apps_slot[fi] = ps ? ps->apps_slot[fi] : -1;
     gs.valid[fi] = family_args(B, li, fid, span, use_masks, k0, kc, base + off, use_masks ? ps->pfs[fi].pre : nullptr,
                                use_masks ? ps->pfs[fi].items : nullptr, use_masks ? ps->pfs[fi].act : nullptr,
                                &gs.fa[fi], &gs.grid[fi], &gs.lds[fi]);
-    off += (size_t)pitch * (fold ? F.n_t : F.num_slots);
+    off += (size_t)pitch * (ffold ? F.n_t : F.num_slots);
   }
   float* inside = (float*)inside_.p;
   float* outside = (float*)outside_.p;
@@ -3048,7 +3061,6 @@ size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span,
   ra.cell_offsets = B.d_cell_offsets;
   ra.lengths = B.d_lengths;
   ra.cell_list = B.d_cells + B.span_start[span] + k0;
-  ra.fold = fold ? 1 : 0;
   ra.off = fold ? R.d_off_f : R.d_off;
   ra.fam = fold ? R.d_fam_f : R.d_fam;
   ra.slot = fold ? R.d_slot_f : R.d_slot;
@@ -3080,12 +3092,28 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
       ps = &(*planned)[si];
     }
     GroupSlice gs;
-    const size_t used = group_slice_args(B, li, rid, span, use_masks, ps, k0, kc, (float*)partial_.p, &gs);
-    // Folded rows start at the encoded semiring zero; every slot of a target maxes into them. (A
-    // group whose families have no items at this span uses no rows.)
-    if (gs.ra.fold && used > 0)
-      hipLaunchKernelGGL(k_fill_bits, dim3((unsigned)std::min<size_t>((used + 255) / 256, 4096)), dim3(256), 0,
-                         stream_, (uint32_t*)partial_.p, used, kEncNegInf);
+    group_slice_args(B, li, rid, span, use_masks, ps, k0, kc, (float*)partial_.p, &gs);
+    // Folded rows start at the encoded semiring zero; every slot of a target maxes into them. Only
+    // the values the rule kernels can write are filled: a planned family's planned item count (on
+    // the device), else the row's items.
+    {
+      FoldFillArgs ff{};
+      int32_t max_rows = 0;
+      for (size_t fi = 0; fi < nfam; ++fi) {
+        if (gs.ra.dmask[fi] == 0 || !gs.valid[fi]) continue;
+        const int k = ff.nfam++;
+        ff.base[k] = gs.partial[fi];
+        ff.pre[k] = use_masks ? ps->pfs[fi].pre : nullptr;
+        ff.num_cells[k] = kc;
+        ff.len[k] = (int32_t)gs.row_items[fi];
+        ff.rows[k] = L.fam[gs.fid[fi]].n_t;
+        ff.pitch[k] = gs.ra.pitch[fi];
+        max_rows = std::max(max_rows, ff.rows[k]);
+      }
+      if (ff.nfam > 0)
+        hipLaunchKernelGGL(k_fill_fold, dim3((unsigned)std::min(max_rows, 1024), (unsigned)ff.nfam), dim3(256), 0,
+                           stream_, ff);
+    }
     if (!persistent) {
       for (size_t fi = 0; fi < nfam; ++fi)
         launch_family(B, li, gs.fid[fi], span, use_masks, k0, kc, gs.partial[fi], use_masks ? ps->pfs[fi].pre : nullptr,

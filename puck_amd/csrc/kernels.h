@@ -85,9 +85,11 @@ struct RedArgs {
   const int32_t* cell_list;
   const int32_t* off;
   const int32_t* fam;
-  const int32_t* slot;  // partial row of each entry: a slot id, or with `fold` the target itself
-  int32_t fold;         // partial values are order-preserving encodings (see FamArgs::fold)
+  const int32_t* slot;  // partial row of each entry: a slot id, or for a folding family the target
   const float* partial[kMaxGroupFamilies];
+  // Per family: 0x7fffffff when its partial rows fold (values are order-preserving encodings, see
+  // FamArgs::fold), else 0.
+  int32_t dmask[kMaxGroupFamilies];
   const int32_t* prefix[kMaxGroupFamilies];  // slice item prefix, null = uniform count
   int32_t uniform_count[kMaxGroupFamilies];
   int32_t pitch[kMaxGroupFamilies];
@@ -105,6 +107,19 @@ struct RedArgs {
   int32_t num_cells, cell_size, n_t, t_half, span;
   int32_t root_init, root;
   float one;
+};
+
+// Start value of folded partial rows: rows [0, rows) x values [0, len) of each family's region get
+// the encoded semiring zero, len = pre[num_cells] for planned slices (their item count is on the
+// device) and `len` otherwise.
+struct FoldFillArgs {
+  float* base[kMaxGroupFamilies];
+  const int32_t* pre[kMaxGroupFamilies];
+  int32_t num_cells[kMaxGroupFamilies];
+  int32_t len[kMaxGroupFamilies];
+  int32_t rows[kMaxGroupFamilies];
+  int32_t pitch[kMaxGroupFamilies];
+  int32_t nfam;
 };
 
 // Device-scheduled pass (k_pass): every (reduce group, span) of one pass becomes two stages, its
