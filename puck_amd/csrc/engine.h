@@ -67,8 +67,8 @@ struct DevFamily {
   int32_t t_half = 0, a_half = 0, b_half = 0;  // ChartHalf of target and operands
   int32_t n_t = 0, n_a = 0, n_b = 0;           // symbol space sizes
   bool unary = false;
-  // Binary families fold their partial rows per target when the layer folds (unary families, with
-  // only a few operand chunks per target, keep one row per slot).
+  // Binary families with at least kMinFoldChunks operand chunks fold their partial rows per target
+  // when the layer folds; unary families and binary ones with few chunks keep one row per slot.
   bool foldable = false;
   int32_t num_chunks = 0, num_slots = 0, num_rules = 0, max_rows = 0;
   std::vector<int32_t> h_chunk_range;       // 4 per chunk: a_lo, a_hi, b_lo, b_hi

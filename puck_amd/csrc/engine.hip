@@ -1743,7 +1743,6 @@ void Layer::build_family(int32_t id) {
   }
   F.num_rules = (int32_t)rules.size();
   F.rule_apps_per_item = (double)rules.size();
-  F.foldable = !F.unary;
 
   // Chunking of the operand symbol spaces so that one tile's rows fit the LDS budget (one
   // workgroup per CU with two items per lane, two workgroups per CU with one).
@@ -1952,6 +1951,9 @@ void Layer::build_family(int32_t id) {
     for (int u = 0; u < kGroupRules; ++u)
       std::memcpy(&packed[(size_t)(F.sentinel_group + k) * gw + 2 * kGroupRules + u], &null_q, 4);
   F.num_slots = (int32_t)F.h_slot_target.size();
+  // Folding pays where a target's rules spread over several operand chunks (the wsj_6 shape: 12-20
+  // chunks); with one or two there is little to fold and atomics cost more than stores.
+  F.foldable = !F.unary && F.num_chunks >= kMinFoldChunks;
   F.d_chunk_range = upload(F.h_chunk_range);
   {
     // One padding entry: a plain stream's rule loop loads the target of the slot after its last.
@@ -3119,15 +3121,16 @@ void Parser::run_group(Batch& B, int32_t li, int32_t rid, int32_t span, bool use
         launch_family(B, li, gs.fid[fi], span, use_masks, k0, kc, gs.partial[fi], use_masks ? ps->pfs[fi].pre : nullptr,
                       use_masks ? ps->pfs[fi].items : nullptr, use_masks ? ps->pfs[fi].act : nullptr, gs.apps_slot[fi]);
     } else {
-      // One queue launch per layout among the group's families (unary, block-major, plain).
-      for (int layout = 0; layout < 3; ++layout) {
+      // One queue launch per layout among the group's families (unary, block-major, plain) and per
+      // partial-row form (folded or one row per slot: the kernel variant differs).
+      for (int layout = 0; layout < 6; ++layout) {
         std::vector<int32_t> fids, slots;
         std::vector<float*> parts;
         std::vector<const int32_t*> pcs, pis;
         std::vector<const uint32_t*> pas;
         for (size_t fi = 0; fi < nfam; ++fi) {
           const DevFamily& F = L.fam[R.families[fi]];
-          const int lay = F.unary ? 0 : (F.skip_layout ? 1 : 2);
+          const int lay = 2 * (F.unary ? 0 : (F.skip_layout ? 1 : 2)) + (gs.ra.dmask[fi] != 0 ? 1 : 0);
           if (lay != layout) continue;
           fids.push_back(R.families[fi]);
           parts.push_back(gs.partial[fi]);
