@@ -379,10 +379,10 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // group of eight rules for the whole wave).
   typedef uint32_t W16 __attribute__((ext_vector_type(16)));
   typedef uint32_t W8 __attribute__((ext_vector_type(8)));
-  typedef uint32_t W4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t W2 __attribute__((ext_vector_type(2)));
   typedef const __attribute__((address_space(4))) W16 ConstW16;
   typedef const __attribute__((address_space(4))) W8 ConstW8;
-  typedef const __attribute__((address_space(4))) W4 ConstW4;
+  typedef const __attribute__((address_space(4))) W2 ConstW2;
   typedef const __attribute__((address_space(4))) char ConstByte;
   typedef const __attribute__((address_space(4))) int32_t ConstI32;
   // Groups are addressed by 32-bit byte offsets from the start of the family's stream.
@@ -390,8 +390,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   ConstByte* gbase = (ConstByte*)a.rules;
   auto group_off = [&](uint32_t o) { return *(ConstW16*)(gbase + o); };
   auto group_q = [&](uint32_t o) { return *(ConstW8*)(gbase + o + 64); };
-  // Skip layouts: ids of the slots ending with each half group (x, y) and their targets (z, w).
-  auto group_slots = [&](uint32_t o) { return *(ConstW4*)(gbase + o + 96); };
+  auto group_slots = [&](uint32_t o) { return *(ConstW2*)(gbase + o + 96); };
 
   // Block ranges of this wave's stream, visited in order. A skip layout lets a masked tile pass
   // over the blocks that none of its items allows; the cursor (cb, c_end) belongs to the furthest
@@ -438,12 +437,6 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     if (gi >= g_end) return;
     g1 = gi + GB;
   }
-  // Folded partial rows (Viterbi): each flush folds into its slot's target row. Plain streams end
-  // their slots in slot-id order, so the next slot's target is loaded one flush ahead (the table
-  // has a padding entry past the last slot); skip layouts carry the targets in their groups.
-  ConstI32* slot_tgt = (ConstI32*)a.slot_target;
-  uint32_t next_tgt = 0;
-  if constexpr (!SKIP && FOLD) next_tgt = (uint32_t)slot_tgt[slot_ctr];
 
   // Per-item scale, kept in registers for the slot flushes.
   float scl[IPL];
@@ -500,12 +493,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     }
     if constexpr (FOLD) {
       {
-        uint32_t t = tgt;
-        if constexpr (!SKIP) {
-          t = next_tgt;
-          next_tgt = (uint32_t)slot_tgt[++slot_ctr];
-        }
-        float* out = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)t * pitch_v;
+        // Folded rows: the slot's target (from its tag) names the row.
+        float* out = reinterpret_cast<float*>(((uint64_t)out_hi << 32) | out_lo) + (uint64_t)tgt * pitch_v;
         if constexpr (IPL == 2 && !SKIP) {
           if (__builtin_amdgcn_readfirstlane(pair_v)) {
             const int m = (tile0 >> 1) + lane;
@@ -541,8 +530,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       if (item < num_items) st_f<COH>(out + item, v[i]);
     }
   };
-  auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, uint32_t tgt, float (*da)[IPL], float (*db)[IPL],
-                  int h) {
+  auto half = [&](const W8& qs, uint32_t tag, uint32_t slot, float (*da)[IPL], float (*db)[IPL], int h) {
 #pragma unroll
     for (int u = h * H; u < h * H + H; ++u) {
       const float q = __uint_as_float(qs[u]);
@@ -560,7 +548,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
       }
     }
     if (tag & kSlotEnd) {
-      flush(slot, tgt);
+      flush(slot, (tag >> kTagTargetShift) & kTagTargetMask);
     }
   };
   // Software pipeline over groups: while group gi is computed from (da, db) with weights cur_q
@@ -569,8 +557,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // loaded into SGPRs. Scalar loads share the LDS wait counter, so the one wait the compiler
   // places before the next step's LDS reads covers both. Register sets alternate between two
   // unrolled steps, so nothing is copied.
-  auto step = [&](const W8& cur_q, const W4& cur_s, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
-                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q, W4& nxt_s,
+  auto step = [&](const W8& cur_q, const W2& cur_s, uint32_t cur_t0, uint32_t cur_t1, const W16& nxt_off,
+                  uint32_t& nxt_t0, uint32_t& nxt_t1, W16& nn_off, W8& nxt_q, W2& nxt_s,
                   float (*da)[IPL], float (*db)[IPL], float (*la)[IPL], float (*lb)[IPL]) {
     // Wait for the previous step's LDS reads and scalar loads before this step's reads issue; a
     // wait the compiler places later (before the first use of da/db) would drain them as well.
@@ -583,8 +571,8 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     nxt_q = group_q(g1);
     if constexpr (SKIP) nxt_s = group_slots(g1);
     __builtin_amdgcn_sched_barrier(0);
-    half(cur_q, cur_t0, cur_s.x, cur_s.z, da, db, 0);
-    half(cur_q, cur_t1, cur_s.y, cur_s.w, da, db, 1);
+    half(cur_q, cur_t0, cur_s.x, da, db, 0);
+    half(cur_q, cur_t1, cur_s.y, da, db, 1);
     __builtin_amdgcn_sched_barrier(0);
     gi = g1;
     g1 = g2;
@@ -594,7 +582,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   auto done = [&]() { return SKIP ? gi == sentinel : gi >= g_end; };
   W16 oA = group_off(gi), oB = group_off(g1);
   W8 qA = group_q(gi), qB;
-  W4 sA = {0u, 0u, 0u, 0u}, sB = {0u, 0u, 0u, 0u};
+  W2 sA = {0u, 0u}, sB = {0u, 0u};
   if constexpr (SKIP) sA = group_slots(gi);
   uint32_t tA0 = oA[H - 1], tA1 = oA[G - 1], tB0 = 0, tB1 = 0;
   float a0[G][IPL], b0[G][IPL], a1[G][IPL], b1[G][IPL];
@@ -1789,6 +1777,7 @@ void Layer::build_family(int32_t id) {
     F.d_block_masks = upload(masks);
   }
 
+  if ((uint32_t)F.n_t > kTagTargetMask + 1u) throw std::runtime_error("more than 8,192 target symbols in one family");
   // Rule stream. Per chunk, the target symbols (slots) are assigned to the workgroup's waves by
   // longest-processing-time-first over their padded rule counts, and each wave streams its slots
   // back to back. A slot's rules are padded with null rules (semiring-zero weight) to a multiple of
@@ -1877,7 +1866,7 @@ void Layer::build_family(int32_t id) {
               stream.push_back(u);
             }
             for (size_t m = j - i; m < padded(j - i); ++m) stream.push_back(null_rule());
-            stream.back().tag = kSlotEnd | ((uint32_t)bot[t] & 127u) << 24;
+            stream.back().tag = kSlotEnd | (uint32_t)t << kTagTargetShift;
             stream.back().slot = slot_id;
           }
           if (block_major || br == nbr - 1)
@@ -1932,10 +1921,7 @@ void Layer::build_family(int32_t id) {
             words[u] = r.x | r.tag;
             words[kGroupRules + u] = r.y;
             words[2 * kGroupRules + u] = r.q;
-            if (r.tag && block_major) {
-              words[3 * kGroupRules + u / kRuleUnroll] = (uint32_t)r.slot;
-              words[3 * kGroupRules + 2 + u / kRuleUnroll] = (uint32_t)F.h_slot_target[r.slot];
-            }
+            if (r.tag && block_major) words[3 * kGroupRules + u / kRuleUnroll] = (uint32_t)r.slot;
           }
           packed.insert(packed.end(), words, words + gw);
         }
@@ -1955,12 +1941,7 @@ void Layer::build_family(int32_t id) {
   // chunks); with one or two there is little to fold and atomics cost more than stores.
   F.foldable = !F.unary && F.num_chunks >= kMinFoldChunks;
   F.d_chunk_range = upload(F.h_chunk_range);
-  {
-    // One padding entry: a plain stream's rule loop loads the target of the slot after its last.
-    std::vector<int32_t> st = F.h_slot_target;
-    st.push_back(0);
-    F.d_slot_target = upload(st);
-  }
+  F.d_slot_target = upload(F.h_slot_target);
   std::vector<double> block_rules(std::max(1, F.unary ? 1 : F.num_blocks), 0.0);
   for (auto& r : rules) block_rules[F.unary ? 0 : std::min(bot[r.t], (int)block_rules.size() - 1)] += 1.0;
   // The blocks with most rules key the mask plan's cell order.
@@ -2346,7 +2327,6 @@ bool Parser::family_args(Batch& B, int32_t li, int32_t fid, int32_t span, bool u
   a.rules = F.d_rules;
   a.block_of_target = F.unary ? nullptr : F.d_block_of_target;
   a.partial = partial;
-  a.slot_target = F.d_slot_target;
   a.fold = folds(L) && F.foldable ? 1 : 0;
   const int items_tile = 64 * L.ipl;
   a.pitch = (int32_t)(((a.pair_splits ? nitems / 2 : nitems) + 255) / 256 * 256);

@@ -14,15 +14,16 @@ constexpr int kRuleUnroll = 4;        // slot rule lists are padded to a multipl
 constexpr int kGroupRules = 8;        // rules per pipelined group of the rule stream
 // A group is stored as kGroupWords dwords: A-row words [0, 8), B-row byte offsets [8, 16), weight
 // bits [16, 24). An A-row word holds the byte offset in bits [0, 18); the words of rules 3 and 7
-// (the last rule of each half group) carry the half group's tag in bits [24, 32): kSlotEnd when
-// a slot ends there, with the slot's block bit in bits [24, 31). Streams that tiles may skip
+// (the last rule of each half group) carry the half group's tag in bits [18, 32): kSlotEnd when
+// a slot ends there, with the slot's target symbol in bits [18, 31). Streams that tiles may skip
 // through (block-major layouts) use kSkipGroupWords: the same words plus, at [24] and [25], the
-// id of the slot ending with each half group, at [26] and [27] that slot's target symbol, and
-// padding.
+// id of the slot ending with each half group, and padding.
 constexpr int kGroupWords = 24;
 constexpr int kSkipGroupWords = 32;
 constexpr uint32_t kOffsetMask = (1u << 18) - 1;
 constexpr uint32_t kSlotEnd = 0x80000000u;
+constexpr int kTagTargetShift = 18;
+constexpr uint32_t kTagTargetMask = (1u << 13) - 1;  // targets of a family: at most 8,191
 // A launch may cut every wave's rule stream into P = 1, 2, 4, ..., kMaxPieces pieces (grid z),
 // at precomputed places no slot spans, so that few tiles still spread over the whole chip.
 constexpr int kMaxPieces = 16;
@@ -54,10 +55,9 @@ struct FamArgs {
   const uint32_t* rules;  // rule groups, kGroupWords dwords each
   const int32_t* block_of_target;
   // Partial rows: one per slot (row = slot id, plain stores), or with `fold` one per target symbol
-  // (row = the slot's target), into which every slot of that target folds its values by an
-  // integer atomic max on order-preserving encodings (see enc_ordered); Viterbi semiring only.
+  // (row = the slot's target, from its tag), into which every slot of that target folds its values
+  // by an integer atomic max on order-preserving encodings (see enc_ordered); Viterbi only.
   float* partial;
-  const int32_t* slot_target;  // target symbol of each slot (one padding entry past the last)
   int32_t fold;
   int32_t num_cells, num_items, uniform_count, slice_cell0;
   int32_t num_block_ranges, sentinel_group;
