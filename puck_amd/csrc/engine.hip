@@ -189,6 +189,10 @@ struct RulesLds {
   int32_t cell_a[ITEMS], cell_b[ITEMS];
   float scale[ITEMS];
   uint32_t uni[4];  // blocks active in any item of the tile (block-major streams)
+  // Prefix-counted slices: item prefix of the cells from the tile's first cell on (slice-relative)
+  // and that cell.
+  int32_t pre[2 * ITEMS + 2];
+  int32_t pre_c0;
 };
 
 // One work unit of a rule family: tile `tile` of ITEMS items x operand chunk `chunk` x piece
@@ -226,6 +230,32 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   float* lds_a = smem;
   float* lds_b = smem + rows_a * a.row_stride;
 
+  // Prefix-counted (unplanned) slices: wave 0 finds the tile's first cell by a 64-way search (a few
+  // dependent loads where a binary search per item takes one per halving) and copies the prefix
+  // of the 2 ITEMS + 2 cells from there on into LDS, where each item finds its cell. Cells without
+  // items (at most one per sentence) are why the window holds two cells per item; an item past
+  // the window falls back to searching the slice.
+  constexpr int kWin = 2 * ITEMS + 2;
+  const bool pre_win = a.prefix != nullptr && a.plan_items == nullptr;
+  if (pre_win) {
+    if (tid < 64) {
+      int lo = 0, hi = a.num_cells;  // largest c in [lo, hi) with prefix[c] - item_begin <= tile0
+      while (hi - lo > 64) {
+        const int step = (hi - lo + 63) >> 6;
+        const int c = lo + tid * step;
+        const uint64_t m = __ballot(c < hi && a.prefix[c] - item_begin <= tile0);
+        lo += (63 - __clzll(m)) * step;  // lane 0 (c = lo) always qualifies
+        hi = min(lo + step, hi);
+      }
+      const int c = lo + tid;
+      const uint64_t m = __ballot(c < hi && a.prefix[c] - item_begin <= tile0);
+      const int c0 = lo + (63 - __clzll(m));
+      for (int i = tid; i < kWin; i += 64) sh.pre[i] = a.prefix[min(c0 + i, a.num_cells)] - item_begin;
+      if (tid == 0) sh.pre_c0 = c0;
+    }
+    __syncthreads();
+  }
+
   if (tid < ITEMS) {
     const int item = tile0 + tid;
     // An invalid item (past the slice, or the missing partner of a lone last split pair) keeps
@@ -262,14 +292,25 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
         k = bx * 64 + (r & 63);
         valid = k < a.num_cells;
       } else {
-        const int abs_item = item + item_begin;
-        int lo = 0, hi = a.num_cells;  // largest k with prefix[k] <= abs_item
+        // Largest i with pre[i] <= item in the LDS window (pre[0] <= tile0 <= item).
+        int lo = 0, hi = kWin;
         while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
-          if (a.prefix[mid] <= abs_item) lo = mid; else hi = mid;
+          const int mid = (lo + hi) >> 1;
+          if (sh.pre[mid] <= item) lo = mid; else hi = mid;
         }
-        k = lo;
-        j = abs_item - a.prefix[k];
+        k = sh.pre_c0 + lo;
+        j = item - sh.pre[lo];
+        if (lo == kWin - 1) {
+          // Possibly past the window: search the slice's prefix from there.
+          const int abs_item = item + item_begin;
+          int l2 = k, h2 = a.num_cells;  // largest k with prefix[k] <= abs_item
+          while (h2 - l2 > 1) {
+            const int mid = (l2 + h2) >> 1;
+            if (a.prefix[mid] <= abs_item) l2 = mid; else h2 = mid;
+          }
+          k = l2;
+          j = abs_item - a.prefix[k];
+        }
       }
     }
     if (valid) {
