@@ -31,7 +31,7 @@ DEFAULTS = dict(streams=3, bucket_cells=1 << 22, persistent=True, graphs=True, p
                 reduce_targets=4)
 SPACE = dict(
     streams=[1, 2, 3, 4],
-    bucket_cells=[1 << 20, 1 << 21, 1 << 22, 1 << 23],
+    bucket_cells=[1 << 18, 1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23],
     piece_target=[64, 128, 256, 512, 1024, 2048],
     items_per_lane=[1, 2],
     reduce_targets=[1, 2, 4, 8, 16],
