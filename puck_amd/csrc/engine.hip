@@ -934,7 +934,18 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
     // lane reads its own items straight from the partial rows, four slots of a family at a time.
     // The target's entries (family, slot) are read 64 at a time with one vector load each and taken
     // lane by lane.
-    const int e0 = a.off[t], ne = a.off[t + 1] - e0;
+    // Whether the target's block is active in the lane's cell, for every family at once (their
+    // loads in flight together); a wave none of whose cells has the target active in any family
+    // (most of them under pruning) is done after this one round of loads.
+    uint32_t use_bits = 0u;
+#pragma unroll
+    for (int g = 0; g < kMaxGroupFamilies; ++g) {
+      if (g < a.nfam && my_k < kend) {
+        const int blk = a.block_of_target[g] != nullptr ? a.block_of_target[g][t] : 0;
+        use_bits |= ((a.act[g][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u) << g;
+      }
+    }
+    const int e0 = a.off[t], ne = __ballot(use_bits != 0u) != 0ull ? a.off[t + 1] - e0 : 0;
     for (int b0 = 0; b0 < ne; b0 += 64) {
       const int cnt = min(64, ne - b0);
       const int li = e0 + b0 + min(lane, cnt - 1);
@@ -943,8 +954,7 @@ __device__ __forceinline__ void reduce_unit(const RedArgs& a, int bx, int by, fl
         const int f = __builtin_amdgcn_readlane(lf, k);
         int kr = k + 1;
         while (kr < cnt && __builtin_amdgcn_readlane(lf, kr) == f) ++kr;
-        const int blk = a.block_of_target[f] != nullptr ? a.block_of_target[f][t] : 0;
-        const bool use = my_k < kend && ((a.act[f][(size_t)(blk >> 5) * a.act_stride + my_c] >> (blk & 31)) & 1u);
+        const bool use = (use_bits >> f) & 1u;
         if (__ballot(use) != 0ull && use) {
           const int32_t* pre = a.prefix[f];
           const int pos = a.posof[f] != nullptr ? a.posof[f][my_c] : my_k;
@@ -3107,6 +3117,7 @@ size_t Parser::group_slice_args(Batch& B, int32_t li, int32_t rid, int32_t span,
   ra.root_init = (rid == kROutBin);
   ra.root = g.root;
   ra.one = g.semiring == kReal ? 1.0f : 0.0f;
+  ra.nfam = (int32_t)nfam;
   gs.rgrid = dim3((kc + 63) / 64, (R.n_t + reduce_targets_ - 1) / reduce_targets_);
   return off;
 }

@@ -108,6 +108,7 @@ struct RedArgs {
   int32_t num_cells, cell_size, n_t, t_half, span;
   int32_t root_init, root;
   float one;
+  int32_t nfam;  // families of the group (entries name families 0 .. nfam - 1)
 };
 
 // Start value of folded partial rows: rows [0, rows) x values [0, len) of each family's region get
