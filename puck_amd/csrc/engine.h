@@ -121,6 +121,11 @@ class Layer {
   bool final_layer;
   bool masked;
   int32_t ipl;  // items per lane of the rule kernels (1 or 2)
+  // Waves per rule workgroup: 16, or 8 for unmasked layers whose every family's operand rows fit
+  // one chunk in half the LDS (two workgroups per CU then overlap one's unit prologue with the
+  // other's rules: the wsj_1 shape).
+  int32_t waves = 16;
+  int32_t lds_budget = 0;  // bytes of operand rows per workgroup
   DevFamily fam[kNumFamilies];
   DevReduce red[kNumReduce];
   PlanFamStatic* d_plan_static = nullptr;  // mask plan data of every family (kNumFamilies)

@@ -199,10 +199,11 @@ struct RulesLds {
 // `piece` of every wave's rule stream. Every thread of the workgroup calls it with the same
 // arguments; it contains workgroup barriers before the rule loop only. COH: operand rows are read
 // and partial rows written with agent-coherent accesses (see ld_f).
-template <int IPL, int SEMI, bool UNARY, bool SKIP, bool COH = false, bool FOLD = false>
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool COH = false, bool FOLD = false, int WAVES = 16>
 __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk, int piece, float* smem,
                                            RulesLds<64 * IPL>& sh) {
   static_assert(!FOLD || SEMI == kViterbi, "only max-plus partials fold by atomic max");
+  static_assert(WAVES == 16 || WAVES == 8, "workgroups of 16 or 8 waves");
   constexpr int ITEMS = 64 * IPL;
   int32_t* s_cell_a = sh.cell_a;
   int32_t* s_cell_b = sh.cell_b;
@@ -363,45 +364,49 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   }
   __syncthreads();
 
-  // Stage operand rows: wave w copies items w, w + 16, ... of the tile; its lanes take 64
-  // consecutive rows of an item's cell at a time, so each load is one contiguous run of the cell's
-  // rows from a wave-uniform base, and one row block's loads for all the wave's items are in flight
-  // together. An invalid item (cell -1) stages the semiring zero.
-  constexpr int kStageWaves = kBlockThreads / 64;
+  // Stage operand rows: wave w copies items w, w + WAVES, ... of the tile, eight at a time; its
+  // lanes take 64 consecutive rows of an item's cell at a time, so each load is one contiguous run
+  // of the cell's rows from a wave-uniform base, and one row block's loads for the batch's items
+  // are in flight together. An invalid item (cell -1) stages the semiring zero.
+  constexpr int kStageWaves = WAVES;
   constexpr int IPW = ITEMS / kStageWaves;  // items per wave
-  static_assert(ITEMS % kStageWaves == 0, "items split evenly over the waves");
+  constexpr int IPB = IPW < 8 ? IPW : 8;    // items per batch
+  static_assert(ITEMS % kStageWaves == 0 && IPW % IPB == 0, "items split evenly over the waves");
   auto stage = [&](const float* ch, const int32_t* cells, int lo, int rows, float* dst) {
     const int sw = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t sl = tid & 63;
-    const char* base[IPW];
-    bool ok[IPW];
 #pragma unroll
-    for (int k = 0; k < IPW; ++k) {
-      const int c = __builtin_amdgcn_readfirstlane(cells[sw + kStageWaves * k]);
-      ok[k] = c >= 0;
-      base[k] = reinterpret_cast<const char*>(ch + (size_t)(ok[k] ? c : 0) * a.cell_size + lo);
-    }
-    // Two row blocks per step: 2 x IPW loads in flight before the first LDS store.
-    for (int r0 = 0; r0 < rows; r0 += 128) {
-      const uint32_t row0 = (uint32_t)r0 + sl, row1 = row0 + 64u;
-      const bool in0 = row0 < (uint32_t)rows, in1 = row1 < (uint32_t)rows;
-      // 32-bit byte offsets from the uniform bases: one load with a scalar base per item. Lanes past
-      // the last row load that row (no branch between the loads) and store nothing.
-      const uint32_t b0 = min(row0, (uint32_t)rows - 1u) * 4u, b1 = min(row1, (uint32_t)rows - 1u) * 4u;
-      float v0[IPW], v1[IPW];
+    for (int h = 0; h < IPW; h += IPB) {
+      const char* base[IPB];
+      bool ok[IPB];
 #pragma unroll
-      for (int k = 0; k < IPW; ++k) v0[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b0));
-#pragma unroll
-      for (int k = 0; k < IPW; ++k) v1[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b1));
-      if (in0) {
-        float* d = dst + row0 * a.row_stride + sw;
-#pragma unroll
-        for (int k = 0; k < IPW; ++k) d[kStageWaves * k] = ok[k] ? v0[k] : zero;
+      for (int k = 0; k < IPB; ++k) {
+        const int c = __builtin_amdgcn_readfirstlane(cells[sw + kStageWaves * (h + k)]);
+        ok[k] = c >= 0;
+        base[k] = reinterpret_cast<const char*>(ch + (size_t)(ok[k] ? c : 0) * a.cell_size + lo);
       }
-      if (in1) {
-        float* d = dst + row1 * a.row_stride + sw;
+      // Two row blocks per step: 2 x IPB loads in flight before the first LDS store.
+      for (int r0 = 0; r0 < rows; r0 += 128) {
+        const uint32_t row0 = (uint32_t)r0 + sl, row1 = row0 + 64u;
+        const bool in0 = row0 < (uint32_t)rows, in1 = row1 < (uint32_t)rows;
+        // 32-bit byte offsets from the uniform bases: one load with a scalar base per item. Lanes
+        // past the last row load that row (no branch between the loads) and store nothing.
+        const uint32_t b0 = min(row0, (uint32_t)rows - 1u) * 4u, b1 = min(row1, (uint32_t)rows - 1u) * 4u;
+        float v0[IPB], v1[IPB];
 #pragma unroll
-        for (int k = 0; k < IPW; ++k) d[kStageWaves * k] = ok[k] ? v1[k] : zero;
+        for (int k = 0; k < IPB; ++k) v0[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b0));
+#pragma unroll
+        for (int k = 0; k < IPB; ++k) v1[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b1));
+        if (in0) {
+          float* d = dst + row0 * a.row_stride + sw + kStageWaves * h;
+#pragma unroll
+          for (int k = 0; k < IPB; ++k) d[kStageWaves * k] = ok[k] ? v0[k] : zero;
+        }
+        if (in1) {
+          float* d = dst + row1 * a.row_stride + sw + kStageWaves * h;
+#pragma unroll
+          for (int k = 0; k < IPB; ++k) d[kStageWaves * k] = ok[k] ? v1[k] : zero;
+        }
       }
     }
   };
@@ -414,7 +419,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   const char* lane_base = reinterpret_cast<const char*>(smem) + lane * IPL * 4;
   constexpr int G = kGroupRules;
   constexpr int H = kRuleUnroll;
-  constexpr int kWaves = kBlockThreads / 64;
+  constexpr int kWaves = WAVES;
   // Rule groups are wave-uniform and never written while a kernel runs: reading them through the
   // constant address space lets the compiler use scalar loads into SGPRs (two s_load_dwordx16 per
   // group of eight rules for the whole wave).
@@ -637,11 +642,11 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
 }
 
 // Plain form: one launch per rule family and span, grid (tiles, chunks, pieces).
-template <int IPL, int SEMI, bool UNARY, bool SKIP, bool FOLD>
-__global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool FOLD, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_rules(FamArgs a) {
   extern __shared__ __align__(16) float smem[];
   __shared__ RulesLds<64 * IPL> sh;
-  rules_unit<IPL, SEMI, UNARY, SKIP, false, FOLD>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem, sh);
+  rules_unit<IPL, SEMI, UNARY, SKIP, false, FOLD, WAVES>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem, sh);
 }
 
 // Persistent form: the rule families of one span share one launch of about one workgroup per CU.
@@ -653,8 +658,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules(FamArgs a) {
 // (tile-fastest), so consecutive units share the chunk's rule stream (the coarse-to-fine parse
 // was 0.6% slower chunk-fastest). Planned families take their item counts, hence their tile
 // counts, from the device, so the launch needs no host knowledge of the pruned work.
-template <int IPL, int SEMI, bool UNARY, bool SKIP, bool FOLD>
-__global__ __launch_bounds__(kBlockThreads) void k_rules_q(QueueArgs q) {
+template <int IPL, int SEMI, bool UNARY, bool SKIP, bool FOLD, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_rules_q(QueueArgs q) {
   constexpr int ITEMS = 64 * IPL;
   extern __shared__ __align__(16) float smem[];
   __shared__ RulesLds<ITEMS> sh;
@@ -693,7 +698,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_rules_q(QueueArgs q) {
       piece = rest & ((1 << q.fam[f].pieces_log2) - 1);
       tile = rest >> q.fam[f].pieces_log2;
     }
-    rules_unit<IPL, SEMI, UNARY, SKIP, false, FOLD>(q.fam[f], tile, chunk, piece, smem, sh);
+    rules_unit<IPL, SEMI, UNARY, SKIP, false, FOLD, WAVES>(q.fam[f], tile, chunk, piece, smem, sh);
   }
 }
 
@@ -1616,9 +1621,15 @@ __global__ void k_get_root(float* out, const float* inside, const int32_t* cell_
 // unary, block-major and plain layouts), for setting their LDS limits.
 template <int IPL, int S, bool FD>
 void rule_kernels(std::vector<const void*>& v) {
-  v.insert(v.end(), {(const void*)k_rules<IPL, S, false, false, FD>, (const void*)k_rules<IPL, S, true, false, FD>,
-                     (const void*)k_rules<IPL, S, false, true, FD>, (const void*)k_rules_q<IPL, S, false, false, FD>,
-                     (const void*)k_rules_q<IPL, S, true, false, FD>, (const void*)k_rules_q<IPL, S, false, true, FD>});
+  v.insert(v.end(), {(const void*)k_rules<IPL, S, false, false, FD, 16>, (const void*)k_rules<IPL, S, true, false, FD, 16>,
+                     (const void*)k_rules<IPL, S, false, true, FD, 16>, (const void*)k_rules_q<IPL, S, false, false, FD, 16>,
+                     (const void*)k_rules_q<IPL, S, true, false, FD, 16>, (const void*)k_rules_q<IPL, S, false, true, FD, 16>});
+}
+// The 8-wave instances (unmasked layers: plain and unary streams, two items per lane, no folding).
+template <int S>
+void rule_kernels_8(std::vector<const void*>& v) {
+  v.insert(v.end(), {(const void*)k_rules<2, S, false, false, false, 8>, (const void*)k_rules<2, S, true, false, false, 8>,
+                     (const void*)k_rules_q<2, S, false, false, false, 8>, (const void*)k_rules_q<2, S, true, false, false, 8>});
 }
 
 // Rule kernels: the semiring as above and, for the Viterbi semiring only, whether partial rows fold.
@@ -1662,6 +1673,18 @@ Layer::Layer(Grammar gr, bool final_l, bool masked_l, int32_t items_per_lane)
   if (ipl != 1 && ipl != 2) throw std::runtime_error("items per lane must be 1 or 2");
   if ((int32_t)g.coarse_names.size() > kMaxCoarseSymbols)
     throw std::runtime_error("at most " + std::to_string(kMaxCoarseSymbols) + " coarse symbols are supported");
+  {
+    // A family's operands are two symbol spaces (one for unaries), so 2 max(#N, #T) rows bound
+    // every family's single-chunk tile.
+    const int stride = 64 * ipl + (ipl == 1 ? 1 : 2);
+    const int rows = 2 * std::max(g.num_nonterms(), g.num_terms());
+    if (ipl == 2 && !masked && rows * stride * 4 <= kLdsHalfBudgetBytes) {
+      waves = 8;
+      lds_budget = kLdsHalfBudgetBytes;
+    } else {
+      lds_budget = ipl == 1 ? kLdsBudgetBytes / 2 : kLdsBudgetBytes;
+    }
+  }
   for (int f = 0; f < kNumFamilies; ++f) build_family(f);
   build_reduce(kRInBin, {kFInNN, kFInNT, kFInTN}, g.num_nonterms());
   build_reduce(kRInBin2, {kFInNN, kFInNT, kFInTN, kFInTT}, g.num_nonterms());
@@ -1796,10 +1819,11 @@ void Layer::build_family(int32_t id) {
   F.num_rules = (int32_t)rules.size();
   F.rule_apps_per_item = (double)rules.size();
 
-  // Chunking of the operand symbol spaces so that one tile's rows fit the LDS budget (one
-  // workgroup per CU with two items per lane, two workgroups per CU with one).
+  // Chunking of the operand symbol spaces so that one tile's rows fit the layer's LDS budget (one
+  // 16-wave workgroup per CU with two items per lane; two workgroups per CU with one item per lane
+  // or with 8 waves).
   const int stride = 64 * ipl + (ipl == 1 ? 1 : 2);
-  const int max_rows = (ipl == 1 ? kLdsBudgetBytes / 2 : kLdsBudgetBytes) / (stride * 4);
+  const int max_rows = lds_budget / (stride * 4);
   int ca = 1, cb = 1;
   if (F.unary) {
     ca = (F.n_a + max_rows - 1) / max_rows;
@@ -1849,7 +1873,7 @@ void Layer::build_family(int32_t id) {
   // (see kGroupWords). In masked layers with several blocks the stream of each wave is block-major
   // (blocks padded to whole groups) and a per-wave table of block ranges lets a tile skip the
   // blocks that none of its cells allows.
-  constexpr int kWaves = kBlockThreads / 64;
+  const int kWaves = waves;
   const bool block_major = masked && !F.unary && F.num_blocks > 1;
   F.skip_layout = block_major;
   F.num_block_ranges = block_major ? F.num_blocks : 1;
@@ -2177,6 +2201,9 @@ Parser::Parser(std::vector<std::shared_ptr<Grammar>> layers, int32_t device, int
   rule_kernels<1, kViterbi, true>(fns);
   rule_kernels<1, kReal, false>(fns);
   rule_kernels<1, kLogSum, false>(fns);
+  rule_kernels_8<kViterbi>(fns);
+  rule_kernels_8<kReal>(fns);
+  rule_kernels_8<kLogSum>(fns);
   for (const void* fn : fns) HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   for (const void* fn : {(const void*)k_pass<2, kViterbi, false>, (const void*)k_pass<2, kViterbi, true>,
                          (const void*)k_pass<2, kReal, false>, (const void*)k_pass<2, kReal, true>,
@@ -2435,14 +2462,23 @@ void Parser::launch_family(Batch& B, int32_t li, int32_t fid, int32_t span, bool
   launch_rules(L.g.semiring, a.fold != 0, [&](auto semi, auto fold) {
     constexpr int S = decltype(semi)::value;
     constexpr bool FD = decltype(fold)::value;
-    if (L.ipl == 2) {
-      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<2, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else hipLaunchKernelGGL((k_rules<2, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
+    if (L.waves == 8) {
+      // 8-wave layers are unmasked with one chunk per family: plain streams, no folding.
+      if constexpr (!FD) {
+        const dim3 blk(64 * 8);
+        if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false, false, 8>), grid, blk, lds, stream_, a);
+        else hipLaunchKernelGGL((k_rules<2, S, false, false, false, 8>), grid, blk, lds, stream_, a);
+      } else {
+        throw std::logic_error("8-wave rule layers do not fold");
+      }
+    } else if (L.ipl == 2) {
+      if (F.unary) hipLaunchKernelGGL((k_rules<2, S, true, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<2, S, false, true, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<2, S, false, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, a);
     } else {
-      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<1, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
-      else hipLaunchKernelGGL((k_rules<1, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, a);
+      if (F.unary) hipLaunchKernelGGL((k_rules<1, S, true, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else if (F.skip_layout) hipLaunchKernelGGL((k_rules<1, S, false, true, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, a);
+      else hipLaunchKernelGGL((k_rules<1, S, false, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, a);
     }
   });
   HIP_CHECK(hipGetLastError());
@@ -2495,7 +2531,8 @@ void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids
   if ((size_t)(queue_next_ + 1) * 4 > queue_.bytes) throw std::runtime_error("rule queue counters exhausted");
   q.counter = (int32_t*)queue_.p + queue_next_++;
   const DevFamily& F0 = L.fam[fids[0]];
-  const dim3 grid((unsigned)std::min<int64_t>(units, (int64_t)num_cus_ * queue_wgs_per_cu_));
+  // About one workgroup per CU (two for 8-wave layers).
+  const dim3 grid((unsigned)std::min<int64_t>(units, (int64_t)num_cus_ * queue_wgs_per_cu_ * (16 / L.waves)));
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (timing) {
     t0 = next_event();
@@ -2505,14 +2542,22 @@ void Parser::launch_queue(Batch& B, int32_t li, const std::vector<int32_t>& fids
   launch_rules(L.g.semiring, q.fam[0].fold != 0, [&](auto semi, auto fold) {
     constexpr int S = decltype(semi)::value;
     constexpr bool FD = decltype(fold)::value;
-    if (L.ipl == 2) {
-      if (F0.unary) hipLaunchKernelGGL((k_rules_q<2, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<2, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else hipLaunchKernelGGL((k_rules_q<2, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
+    if (L.waves == 8) {
+      if constexpr (!FD) {
+        const dim3 blk(64 * 8);
+        if (F0.unary) hipLaunchKernelGGL((k_rules_q<2, S, true, false, false, 8>), grid, blk, lds, stream_, q);
+        else hipLaunchKernelGGL((k_rules_q<2, S, false, false, false, 8>), grid, blk, lds, stream_, q);
+      } else {
+        throw std::logic_error("8-wave rule layers do not fold");
+      }
+    } else if (L.ipl == 2) {
+      if (F0.unary) hipLaunchKernelGGL((k_rules_q<2, S, true, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<2, S, false, true, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else hipLaunchKernelGGL((k_rules_q<2, S, false, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, q);
     } else {
-      if (F0.unary) hipLaunchKernelGGL((k_rules_q<1, S, true, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<1, S, false, true, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
-      else hipLaunchKernelGGL((k_rules_q<1, S, false, false, FD>), grid, dim3(kBlockThreads), lds, stream_, q);
+      if (F0.unary) hipLaunchKernelGGL((k_rules_q<1, S, true, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else if (F0.skip_layout) hipLaunchKernelGGL((k_rules_q<1, S, false, true, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, q);
+      else hipLaunchKernelGGL((k_rules_q<1, S, false, false, FD, 16>), grid, dim3(kBlockThreads), lds, stream_, q);
     }
   });
   HIP_CHECK(hipGetLastError());
@@ -2821,6 +2866,7 @@ bool Parser::build_dev_pass(Batch& B, int32_t li, bool outside, bool use_masks, 
   const Batch::PassPlan* pp = static_cast<const Batch::PassPlan*>(pass_plan);
   Batch::DevPass& dp = *static_cast<Batch::DevPass*>(dev_pass);
   Layer& L = *layers_[li];
+  if (L.waves != 16) return false;  // k_pass runs 16-wave rule units
   const auto groups = outside ? outside_groups(B) : inside_groups(B);
   const int64_t half = (int64_t)(partial_.bytes / 4 / 2);
   for (auto [rid, span] : groups) {

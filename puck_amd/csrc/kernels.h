@@ -10,6 +10,8 @@ namespace puck {
 constexpr int kMaxGroupFamilies = 8;
 constexpr int kBlockThreads = 1024;   // rule kernels: 16 waves share one LDS item tile
 constexpr int kLdsBudgetBytes = 150 * 1024;
+// Operand rows of an 8-wave workgroup, two of which share a CU's 160 KB with their static LDS.
+constexpr int kLdsHalfBudgetBytes = 77 * 1024;
 constexpr int kRuleUnroll = 4;        // slot rule lists are padded to a multiple of this
 constexpr int kGroupRules = 8;        // rules per pipelined group of the rule stream
 // A group is stored as kGroupWords dwords: A-row words [0, 8), B-row byte offsets [8, 16), weight
