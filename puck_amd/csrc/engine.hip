@@ -1372,45 +1372,75 @@ __global__ __launch_bounds__(256) void k_mbr(MbrCell* out, const float* inside, 
                                              const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
                                              int32_t num_cells, const int32_t* t_off, const int32_t* t_syms,
                                              const int32_t* n_off, const int32_t* n_syms, int32_t cell_size,
-                                             int32_t root, int32_t num_coarse, int32_t logsum) {
-  const int cell = blockIdx.x * 4 + (threadIdx.x >> 6);
+                                             int32_t root, int32_t num_coarse, int32_t num_syms, int32_t logsum) {
+  // The symbol lists of both halves, once per block, and each wave's per-symbol terms, in LDS.
+  __shared__ int32_t s_off[2][kMaxCoarseSymbols + 1];
+  __shared__ int32_t s_syms[2][kMbrMaxSyms];
+  __shared__ float s_rows[4][kMbrMaxCell];
+  for (int i = threadIdx.x; i <= num_coarse; i += 256) {
+    s_off[0][i] = t_off[i];
+    s_off[1][i] = n_off[i];
+  }
+  for (int i = threadIdx.x; i < num_syms; i += 256) {
+    s_syms[0][i] = t_syms[i];
+    s_syms[1][i] = n_syms[i];
+  }
+  __syncthreads();
+  const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  if (cell >= num_cells) return;
-  int lo = 0, hi = num_sent;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (cell_offsets[mid] <= cell) lo = mid; else hi = mid;
-  }
-  const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
-  (void)lengths;
-  const float root_score = inside[(size_t)(last - 1) * cell_size + root];
-  const bool term = (cell - first) < lengths[sent];
-  const int32_t* off = term ? t_off : n_off;
-  const int32_t* syms = term ? t_syms : n_syms;
-  const float* in = inside + (size_t)cell * cell_size;
-  const float* o = outside + (size_t)cell * cell_size;
-  float best = -INFINITY;
-  int best_c = -1;
-  for (int c = lane; c < num_coarse; c += 64) {
-    float m = 0.0f;
-    for (int k = off[c]; k < off[c + 1]; ++k) {
-      const int sym = syms[k];
-      m += logsum ? expf(in[sym] - root_score + o[sym]) : in[sym] / root_score * o[sym];
+  float* in = s_rows[wv];  // the cell's marginal term per symbol
+  for (int j = 0; j < kMbrCellsPerBlock / 4; ++j) {
+    const int cell = blockIdx.x * kMbrCellsPerBlock + 4 * j + wv;
+    if (cell >= num_cells) return;
+    // Sentence of the cell: largest s with cell_offsets[s] <= cell, by a 64-way ballot search (a
+    // few dependent loads instead of one per halving). Lane 0's probe (s = lo) always qualifies.
+    int lo = 0, hi = num_sent;
+    while (hi - lo > 1) {
+      const int step = (hi - lo + 63) >> 6;
+      const int sidx = lo + lane * step;
+      const uint64_t m = __ballot(sidx < hi && cell_offsets[sidx] <= cell);
+      lo += (63 - __clzll(m)) * step;
+      hi = min(lo + step, hi);
     }
-    if (m > best) {  // lanes visit their coarse symbols in increasing order: first max wins
-      best = m;
-      best_c = c;
+    const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
+    const float root_score = inside[(size_t)(last - 1) * cell_size + root];
+    const int half = (cell - first) < lengths[sent] ? 0 : 1;  // span-1 bottom cells hold tags
+    const int32_t* off = s_off[half];
+    const int32_t* syms = s_syms[half];
+    // The previous cell's reads of the rows are done (LDS operations of a wave complete in order;
+    // the fences keep the compiler from moving accesses across).
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Each symbol's marginal term, computed once across the lanes: in / root * out (Real) or
+    // exp(in - root + out) (LogSum); the coarse sums below add them in symbol order.
+    for (int i = lane; i < cell_size; i += 64) {
+      const float x = inside[(size_t)cell * cell_size + i], y = outside[(size_t)cell * cell_size + i];
+      in[i] = logsum ? expf(x - root_score + y) : x / root_score * y;
     }
-  }
-  for (int w = 32; w > 0; w >>= 1) {
-    const float ob = __shfl_xor(best, w);
-    const int oc = __shfl_xor(best_c, w);
-    if (ob > best || (ob == best && oc >= 0 && (best_c < 0 || oc < best_c))) {
-      best = ob;
-      best_c = oc;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float best = -INFINITY;
+    int best_c = -1;
+    for (int c = lane; c < num_coarse; c += 64) {
+      float m = 0.0f;
+      for (int k = off[c]; k < off[c + 1]; ++k) m += in[syms[k]];
+      if (m > best) {  // lanes visit their coarse symbols in increasing order: first max wins
+        best = m;
+        best_c = c;
+      }
     }
+    for (int w = 32; w > 0; w >>= 1) {
+      const float ob = __shfl_xor(best, w);
+      const int oc = __shfl_xor(best_c, w);
+      if (ob > best || (ob == best && oc >= 0 && (best_c < 0 || oc < best_c))) {
+        best = ob;
+        best_c = oc;
+      }
+    }
+    if (lane == 0) out[cell] = MbrCell{best, best_c};
   }
-  if (lane == 0) out[cell] = MbrCell{best, best_c};
 }
 
 // viterbi (CLViterbi.scala:495-605): top-down best tree, one wave per sentence. The reference scans
@@ -3534,10 +3564,12 @@ void Parser::launch_viterbi(Batch& B, int32_t li) {
 void Parser::launch_mbr(Batch& B, int32_t li) {
   const Layer& L = *layers_[li];
   const Grammar& g = L.g;
-  hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + 3) / 4), dim3(256), 0, stream_,
+  if (g.cell_size > kMbrMaxCell || g.num_syms > kMbrMaxSyms)
+    throw std::runtime_error("MBR decode supports cells of at most 1,024 and grammars of at most 2,048 symbols");
+  hipLaunchKernelGGL(k_mbr, dim3((B.num_cells + kMbrCellsPerBlock - 1) / kMbrCellsPerBlock), dim3(256), 0, stream_,
                      (MbrCell*)mbr_.p, (const float*)inside_.p, (const float*)outside_.p,
                      B.d_cell_offsets, B.d_lengths, B.n, B.num_cells, L.d_mbr_t_off, L.d_mbr_t_syms,
-                     L.d_mbr_n_off, L.d_mbr_n_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size(),
+                     L.d_mbr_n_off, L.d_mbr_n_syms, g.cell_size, g.root, (int32_t)g.coarse_names.size(), g.num_syms,
                      g.semiring == kLogSum);
   HIP_CHECK(hipGetLastError());
 }
