@@ -1312,6 +1312,21 @@ __global__ void k_tags(float* inside, const int32_t* cell_offsets, const int32_t
 }
 
 // computeMasks (CLMaskKernels.scala:172-236): one wave per cell, bits OR-reduced across the wave.
+// Sentence of a chart cell, for a whole wave (every lane active, `cell` wave-uniform): the largest s
+// with cell_offsets[s] <= cell, by a 64-way ballot search, a few dependent loads where a binary
+// search takes one per halving. Lane 0's probe (s = lo) always qualifies.
+__device__ inline int sentence_of(const int32_t* cell_offsets, int num_sent, int cell, int lane) {
+  int lo = 0, hi = num_sent;
+  while (hi - lo > 1) {
+    const int step = (hi - lo + 63) >> 6;
+    const int sidx = lo + lane * step;
+    const uint64_t m = __ballot(sidx < hi && cell_offsets[sidx] <= cell);
+    lo += (63 - __clzll(m)) * step;
+    hi = min(lo + step, hi);
+  }
+  return lo;
+}
+
 __global__ void k_masks(uint32_t* masks_out, const float* inside, const float* outside,
                         const int32_t* cell_offsets, const int32_t* lengths, int32_t num_sent,
                         int32_t num_cells, const int32_t* proj_t, const int32_t* proj_n,
@@ -1320,12 +1335,8 @@ __global__ void k_masks(uint32_t* masks_out, const float* inside, const float* o
   const int cell = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (cell >= num_cells) return;
-  int lo = 0, hi = num_sent;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (cell_offsets[mid] <= cell) lo = mid; else hi = mid;
-  }
-  const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
+  const int sent = sentence_of(cell_offsets, num_sent, cell, lane);
+  const int first = cell_offsets[sent], last = cell_offsets[sent + 1];
   const int n = lengths[sent];
   const float root_score = inside[(size_t)(last - 1) * cell_size + root];
   const float cutoff = root_score + thresh;
@@ -1392,17 +1403,8 @@ __global__ __launch_bounds__(256) void k_mbr(MbrCell* out, const float* inside, 
   for (int j = 0; j < kMbrCellsPerBlock / 4; ++j) {
     const int cell = blockIdx.x * kMbrCellsPerBlock + 4 * j + wv;
     if (cell >= num_cells) return;
-    // Sentence of the cell: largest s with cell_offsets[s] <= cell, by a 64-way ballot search (a
-    // few dependent loads instead of one per halving). Lane 0's probe (s = lo) always qualifies.
-    int lo = 0, hi = num_sent;
-    while (hi - lo > 1) {
-      const int step = (hi - lo + 63) >> 6;
-      const int sidx = lo + lane * step;
-      const uint64_t m = __ballot(sidx < hi && cell_offsets[sidx] <= cell);
-      lo += (63 - __clzll(m)) * step;
-      hi = min(lo + step, hi);
-    }
-    const int sent = lo, first = cell_offsets[sent], last = cell_offsets[sent + 1];
+    const int sent = sentence_of(cell_offsets, num_sent, cell, lane);
+    const int first = cell_offsets[sent], last = cell_offsets[sent + 1];
     const float root_score = inside[(size_t)(last - 1) * cell_size + root];
     const int half = (cell - first) < lengths[sent] ? 0 : 1;  // span-1 bottom cells hold tags
     const int32_t* off = s_off[half];
