@@ -371,6 +371,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   constexpr int kStageWaves = WAVES;
   constexpr int IPW = ITEMS / kStageWaves;  // items per wave
   constexpr int IPB = IPW < 8 ? IPW : 8;    // items per batch
+  constexpr int kStageBlocks = 2;
   static_assert(ITEMS % kStageWaves == 0 && IPW % IPB == 0, "items split evenly over the waves");
   auto stage = [&](const float* ch, const int32_t* cells, int lo, int rows, float* dst) {
     const int sw = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -385,37 +386,52 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
         ok[k] = c >= 0;
         base[k] = reinterpret_cast<const char*>(ch + (size_t)(ok[k] ? c : 0) * a.cell_size + lo);
       }
-      // Two row blocks per step: 2 x IPB loads in flight before the first LDS store.
-      for (int r0 = 0; r0 < rows; r0 += 128) {
-        const uint32_t row0 = (uint32_t)r0 + sl, row1 = row0 + 64u;
-        const bool in0 = row0 < (uint32_t)rows, in1 = row1 < (uint32_t)rows;
-        // 32-bit byte offsets from the uniform bases: one load with a scalar base per item. Lanes
-        // past the last row load that row (no branch between the loads) and store nothing.
-        const uint32_t b0 = min(row0, (uint32_t)rows - 1u) * 4u, b1 = min(row1, (uint32_t)rows - 1u) * 4u;
-        float v0[IPB], v1[IPB];
+      // Up to kStageBlocks row blocks of 64 per step: kStageBlocks x IPB loads in flight before the
+      // first LDS store (an operand's rows of a 150 KB tile take one or two steps).
+      for (int r0 = 0; r0 < rows; r0 += 64 * kStageBlocks) {
+        float v[kStageBlocks][IPB];
 #pragma unroll
-        for (int k = 0; k < IPB; ++k) v0[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b0));
+        for (int q = 0; q < kStageBlocks; ++q) {
+          // 32-bit byte offsets from the uniform bases: one load with a scalar base per item. Lanes
+          // past the last row load that row (no branch between the loads) and store nothing.
+          const uint32_t row = (uint32_t)r0 + 64u * q + sl;
+          const uint32_t b = min(row, (uint32_t)rows - 1u) * 4u;
 #pragma unroll
-        for (int k = 0; k < IPB; ++k) v1[k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b1));
-        if (in0) {
-          float* d = dst + row0 * a.row_stride + sw + kStageWaves * h;
-#pragma unroll
-          for (int k = 0; k < IPB; ++k) d[kStageWaves * k] = ok[k] ? v0[k] : zero;
+          for (int k = 0; k < IPB; ++k) v[q][k] = ld_f<COH>(reinterpret_cast<const float*>(base[k] + b));
         }
-        if (in1) {
-          float* d = dst + row1 * a.row_stride + sw + kStageWaves * h;
 #pragma unroll
-          for (int k = 0; k < IPB; ++k) d[kStageWaves * k] = ok[k] ? v1[k] : zero;
+        for (int q = 0; q < kStageBlocks; ++q) {
+          const uint32_t row = (uint32_t)r0 + 64u * q + sl;
+          if (row < (uint32_t)rows) {
+            float* d = dst + row * a.row_stride + sw + kStageWaves * h;
+#pragma unroll
+            for (int k = 0; k < IPB; ++k) d[kStageWaves * k] = ok[k] ? v[q][k] : zero;
+          }
         }
       }
     }
   };
+  // Block ranges of this wave's stream (skip layouts): lane l loads the bounds of blocks l and
+  // 64 + l here, so the loads are in flight with the staging.
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nbr = a.num_block_ranges;
+  const int32_t* bt = a.wave_blocks + (size_t)(chunk * WAVES + wave) * (nbr + 2 + kPieceTableInts);
+  int bt0[2] = {0, 0}, bt1[2] = {0, 0};
+  if constexpr (SKIP) {
+    if (lane < nbr) {
+      bt0[0] = bt[lane];
+      bt1[0] = bt[lane + 1];
+    }
+    if (64 + lane < nbr) {
+      bt0[1] = bt[64 + lane];
+      bt1[1] = bt[65 + lane];
+    }
+  }
   stage(a.chA, s_cell_a, a_lo, rows_a, lds_a);
   if constexpr (!UNARY) stage(a.chB, s_cell_b, b_lo, rows_b, lds_b);
   __syncthreads();
 
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const char* lane_base = reinterpret_cast<const char*>(smem) + lane * IPL * 4;
   constexpr int G = kGroupRules;
   constexpr int H = kRuleUnroll;
@@ -439,13 +455,11 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   auto group_slots = [&](uint32_t o) { return *(ConstW2*)(gbase + o + 96); };
 
   // Block ranges of this wave's stream, visited in order. A skip layout lets a masked tile pass
-  // over the blocks that none of its items allows; the cursor (cb, c_end) belongs to the furthest
-  // group fetched so far. Otherwise the wave streams one range and counts its slots.
-  const int nbr = a.num_block_ranges;
-  ConstI32* bt = (ConstI32*)a.wave_blocks + (size_t)(chunk * kWaves + wave) * (nbr + 2 + kPieceTableInts);
+  // over the blocks that none of its items allows; the cursor's range end c_end belongs to the
+  // furthest group fetched so far. Otherwise the wave streams one range and counts its slots.
   // This workgroup's piece of the stream: [p_lo, p_hi) in groups, first slot p_slot.
   const int npieces = 1 << a.pieces_log2;
-  ConstI32* sp = bt + nbr + 2 + 2 * ((npieces - 1) + a.pieces_log2) + 2 * piece;
+  ConstI32* sp = (ConstI32*)bt + nbr + 2 + 2 * ((npieces - 1) + a.pieces_log2) + 2 * piece;
   const int p_lo = sp[0], p_slot = sp[1], p_hi = sp[2];
   uint64_t un_lo = ~0ull, un_hi = ~0ull;  // block bits 0-63 and 64-127 of the tile's union
   if (SKIP && a.masks != nullptr) {
@@ -454,18 +468,39 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
     un_hi = (uint64_t)__builtin_amdgcn_readfirstlane(s_union[2]) |
             (uint64_t)__builtin_amdgcn_readfirstlane(s_union[3]) << 32;
   }
-  int cb = -1;
+  // The wave's block ranges, clipped to its piece, in vector registers: lane l holds blocks l and
+  // 64 + l as byte offsets, and a 64-bit mask per half marks the ranges the tile visits (in the
+  // union, non-empty). A range change then takes the next set bit and two lane reads, with no
+  // memory access: a scalar load of the bounds there would wait on the LDS reads in flight too.
   uint32_t c_end = 0;  // byte offset of the end of the cursor's range
   const uint32_t sentinel = (uint32_t)a.sentinel_group * GB;
+  // (Named registers, not arrays: an array indexed in the loop would live in scratch memory.)
+  uint32_t lo_b0 = 0u, lo_b1 = 0u, hi_b0 = 0u, hi_b1 = 0u;
+  uint64_t visit_lo = 0ull, visit_hi = 0ull;
+  if constexpr (SKIP) {
+    const int b0 = max(bt0[0], p_lo), b1 = min(bt1[0], p_hi);
+    lo_b0 = (uint32_t)b0 * GB;
+    lo_b1 = (uint32_t)b1 * GB;
+    visit_lo = __ballot(lane < nbr && b0 < b1) & un_lo;
+    if (nbr > 64) {
+      const int c0 = max(bt0[1], p_lo), c1 = min(bt1[1], p_hi);
+      hi_b0 = (uint32_t)c0 * GB;
+      hi_b1 = (uint32_t)c1 * GB;
+      visit_hi = __ballot(64 + lane < nbr && c0 < c1) & un_hi;
+    }
+  }
   auto next_range = [&]() -> uint32_t {
-    while (++cb < nbr) {
-      const uint64_t word = (cb & 64) ? un_hi : un_lo;
-      if (!((word >> (cb & 63)) & 1u)) continue;
-      const int b0 = max(bt[cb], p_lo), b1 = min(bt[cb + 1], p_hi);
-      if (b0 < b1) {
-        c_end = (uint32_t)b1 * GB;
-        return (uint32_t)b0 * GB;
-      }
+    if (visit_lo != 0ull) {
+      const int l = __builtin_ctzll(visit_lo);
+      visit_lo &= visit_lo - 1ull;
+      c_end = (uint32_t)__builtin_amdgcn_readlane((int)lo_b1, l);
+      return (uint32_t)__builtin_amdgcn_readlane((int)lo_b0, l);
+    }
+    if (visit_hi != 0ull) {
+      const int l = __builtin_ctzll(visit_hi);
+      visit_hi &= visit_hi - 1ull;
+      c_end = (uint32_t)__builtin_amdgcn_readlane((int)hi_b1, l);
+      return (uint32_t)__builtin_amdgcn_readlane((int)hi_b0, l);
     }
     return sentinel;
   };
