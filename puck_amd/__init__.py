@@ -290,10 +290,15 @@ def synth_wsj6_cached(seed=6, cache_dir=None):
     os.makedirs(cache_dir, exist_ok=True)
     prefix = os.path.join(cache_dir, f"wsj6syn_s{seed}.gr")
     done = prefix + ".done"
-    if not (os.path.exists(done) and os.path.exists(prefix + ".hierarchy")):
-        synth_wsj6(prefix, seed)
-        with open(done, "w") as f:
-            f.write("ok\n")
+    # Concurrent processes (pytest workers) serialise on a lock file, so no
+    # reader sees a grammar another process is still writing.
+    import fcntl
+    with open(prefix + ".lock", "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        if not (os.path.exists(done) and os.path.exists(prefix + ".hierarchy")):
+            synth_wsj6(prefix, seed)
+            with open(done, "w") as f:
+                f.write("ok\n")
     return prefix
 
 
