@@ -32,10 +32,10 @@ constexpr int kMaxPieces = 16;
 constexpr int kLog2MaxPieces = 4;
 constexpr int kPieceTableInts = 2 * (2 * kMaxPieces + kLog2MaxPieces);  // sum over P of 2 (P + 1)
 constexpr int kReduceTargets = 16;    // reduce kernel: targets (waves) per workgroup
-constexpr int kMinFoldChunks = 3;
+constexpr int kMinFoldChunks = 3;     // binary families with this many operand chunks fold partials
 constexpr int kMbrMaxCell = 1024;     // largest cell (floats) the MBR kernel stages in LDS
 constexpr int kMbrMaxSyms = 2048;     // largest grammar (symbols) whose MBR symbol lists it stages
-constexpr int kMbrCellsPerBlock = 16; // cells per MBR workgroup (four per wave)     // binary families with this many operand chunks fold partials
+constexpr int kMbrCellsPerBlock = 16; // cells per MBR workgroup (four per wave)
 
 // Arguments of the rule-family kernel (one launch = one family, one span, one cell slice).
 struct FamArgs {
