@@ -370,7 +370,7 @@ __device__ __forceinline__ void rules_unit(const FamArgs& a, int tile, int chunk
   // are in flight together. An invalid item (cell -1) stages the semiring zero.
   constexpr int kStageWaves = WAVES;
   constexpr int IPW = ITEMS / kStageWaves;  // items per wave
-  constexpr int IPB = IPW < 8 ? IPW : 8;    // items per batch
+  constexpr int IPB = IPW < 16 ? IPW : 16;  // items per batch
   constexpr int kStageBlocks = 2;
   static_assert(ITEMS % kStageWaves == 0 && IPW % IPB == 0, "items split evenly over the waves");
   auto stage = [&](const float* ch, const int32_t* cells, int lo, int rows, float* dst) {
