@@ -139,6 +139,11 @@ def plain_vs_batched(puck, layers, words, offsets, steps, cfg, label, outside_fi
            "buckets": int(st["buckets"]), "graph_replays": int(st["graph_replays"]), "batched": cfg,
            "roofline_plain": fine_roofline(launches, kernel_labels[0]),
            "roofline_batched": fine_roofline(launches_q, kernel_labels[1])}
+    hb = pipeline_bytes(puck, layers, lens, outside_final)
+    out["hbm"] = {"algorithmic_bytes_per_parse": hb, "achieved": round(hb / (ms_b / 1e3) / 1e9, 1),
+                  "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(hb / (ms_b / 1e3) / HBM_PEAK, 5),
+                  "note": "every layer's chart passes written and read once, masks written and read, 16 B "
+                          "per tree node; the path is VALU-bound, so this fraction is small"}
     print(f"[bench] {label}: batched {out['value']} vs plain {out['plain']} sentences/s", file=sys.stderr, flush=True)
     return out
 
@@ -154,6 +159,23 @@ def fine_roofline(launches, kernel):
             "unit": "TFLOP/s", "frac": round(ach / VALU_PEAK, 4), "kernel": kernel,
             "launches": int(sel.sum()), "rule_apps": float(apps[sel].sum()),
             "avg_launch_ms": round(float(ms[sel].mean()), 4) if sel.any() else None}
+
+
+def pipeline_bytes(puck, layers, lens, outside_final):
+    """Algorithmic HBM bytes of a coarse-to-fine parse: every layer's chart passes (inside, and
+    outside where it runs: coarse layers, a Real final layer, insideOutside) each write and read
+    their chart once, every coarse layer writes its masks and the next reads them, and the decode
+    writes 16 bytes per tree node."""
+    lens = np.asarray(lens, np.int64)
+    cells = int(np.sum(lens * (lens + 1)))
+    total = 16 * int(np.sum(2 * lens - 1))
+    for k, g in enumerate(layers):
+        last = k == len(layers) - 1
+        passes = 2 if (not last or outside_final or g.semiring != puck.VITERBI) else 1
+        total += passes * 2 * 4 * g.info.cell_size * cells
+        if not last:
+            total += 2 * 4 * g.info.mask_words * cells
+    return total
 
 
 def chart_bytes(lens, cell_size, passes):
