@@ -1,0 +1,47 @@
+"""Writes smallbank_sentences.txt: the word sequences of the reference's own test treebank
+(src/test/resources/smallbank/dev and test, 68 Penn Treebank trees of Wall Street Journal text),
+one sentence per line, tokens separated by spaces, in file order (dev, then test). These are real
+sentences for the text front end and the parse path; the trees themselves are not used (the
+reference trains an epic grammar from them, which is not in the checkout).
+
+Run from the repository root: python tests/golden/make_smallbank.py
+"""
+import os
+import re
+
+REF = "/root/reference/src/test/resources/smallbank"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "smallbank_sentences.txt")
+
+
+def trees(text):
+    """Top-level bracketed trees of a treebank file."""
+    out, depth, start = [], 0, 0
+    for i, c in enumerate(text):
+        if c == "(":
+            if depth == 0:
+                start = i
+            depth += 1
+        elif c == ")":
+            depth -= 1
+            if depth == 0:
+                out.append(text[start:i + 1])
+    return out
+
+
+def words(tree):
+    """The tree's yield: the word of every (tag word) leaf, empty elements (-NONE-) dropped."""
+    return [w for tag, w in re.findall(r"\(([^\s()]+) ([^\s()]+)\)", tree) if tag != "-NONE-"]
+
+
+def main():
+    lines = []
+    for part in ("dev", "test"):
+        with open(os.path.join(REF, part)) as f:
+            lines += [" ".join(words(t)) for t in trees(f.read())]
+    with open(OUT, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"wrote {len(lines)} sentences to {OUT}")
+
+
+if __name__ == "__main__":
+    main()
